@@ -1,0 +1,111 @@
+/*
+ * chuckle_hip.h -- C ABI of the MI355X implementation of Chuckle's
+ * file-to-file search hot path (overlap sweep, coverage moments, analytic
+ * p-values).  Plain C types only; every function returns CHK_OK (0) or a
+ * CHK_ERR_* code and leaves a thread-local message for chk_last_error().
+ * No exception or abort crosses this boundary.
+ *
+ * Handles are bound to one device.  Calls on one handle are serialised on
+ * that handle's HIP stream; different handles may be used from different
+ * threads.
+ *
+ * Reference interfaces replaced (simondoesstuff/riggle, crate `chuckle`):
+ *   chk_index_create      -> the layered on-disk index opened by query_database
+ *                            (src/io/layer.rs:34-105 MappedLayer,
+ *                             src/io/jump.rs:19-52 MappedJumpTable,
+ *                             src/tasks/query/mod.rs:204-241)
+ *   chk_query_create      -> parse_file_batch's per-shard query intervals
+ *                            (src/tasks/query/files.rs:40-111, mod.rs:193-202)
+ *   chk_sweep             -> query_sweep_windowed + the window merge
+ *                            (src/sweep/query.rs:127-160,
+ *                             src/tasks/query/mod.rs:243-298,
+ *                             src/matrix/dense.rs:4-160)
+ */
+#ifndef CHUCKLE_HIP_H
+#define CHUCKLE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHK_OK 0
+#define CHK_ERR_INVALID 1   /* bad argument / malformed input */
+#define CHK_ERR_HIP 2       /* HIP runtime error */
+#define CHK_ERR_NOMEM 3     /* device or host allocation failed */
+#define CHK_ERR_IO 4        /* file read / write failed */
+#define CHK_ERR_NOT_FOUND 5 /* database / file / entry missing */
+#define CHK_ERR_INTERNAL 6
+
+typedef struct chk_index chk_index;
+typedef struct chk_query chk_query;
+
+/* Library version string and the last error message of the calling thread. */
+const char* chk_version(void);
+const char* chk_last_error(void);
+
+/* Number of visible HIP devices. */
+int chk_device_count(int* count);
+
+/*
+ * Build an HBM-resident index from host arrays of n intervals.
+ *   shard[i] < nshards  -- shard (chromosome) id of interval i
+ *   start[i] < end[i] <= 2^31-1  -- half-open coordinates
+ *   fid[i]   < nfiles   -- DB file id (D_SID)
+ * Device layout: per shard, a start stream (int32 start sorted ascending +
+ * u32 file id) and an end stream (int32 end sorted ascending + u32 file id).
+ * Replaces the reference's layer files + jump tables: the merge-path sweep
+ * needs neither size classes nor look-back.
+ */
+int chk_index_create(int device, uint32_t nshards, uint32_t nfiles, uint64_t n,
+                     const uint32_t* shard, const uint32_t* start, const uint32_t* end,
+                     const uint32_t* fid, chk_index** out);
+int chk_index_free(chk_index* idx);
+int chk_index_info(const chk_index* idx, uint32_t* nshards, uint32_t* nfiles, uint64_t* n,
+                   uint64_t* device_bytes);
+
+/*
+ * Upload a batch of query files: n intervals with shard ids in the index's
+ * shard numbering (ids >= nshards mark shards absent from the DB; their
+ * intervals are ignored, as the reference skips such shards), coordinates as
+ * above and qfile[i] < nfiles the query-file row (Q_SID).  qfile may be NULL
+ * when nfiles == 1.
+ */
+int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t* shard,
+                     const uint32_t* start, const uint32_t* end, const uint32_t* qfile,
+                     chk_query** out);
+int chk_query_free(chk_query* q);
+
+/*
+ * Overlap sweep: for every (query file r, DB file f) cell,
+ *   counts[r*F + f] = #{(q, d) : same shard, d.start < q.end && d.end > q.start}
+ *   bp[r*F + f]     = sum over those pairs of |q intersect d| in base pairs,
+ * exact u64 (the reference's observed_bins is bp / 100).  Host outputs of
+ * nfiles(query) x nfiles(index) elements; blocks until done.
+ */
+int chk_sweep(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp);
+
+/* Same, into device buffers (chk_dev_alloc), enqueued on the index's stream
+ * without a final synchronisation. */
+int chk_sweep_async(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_bp);
+
+/* Device memory helpers for callers without a device allocator. */
+int chk_dev_alloc(chk_index* idx, uint64_t bytes, void** ptr);
+int chk_dev_free(chk_index* idx, void* ptr);
+int chk_dev_to_host(chk_index* idx, void* dst, const void* src, uint64_t bytes);
+int chk_sync(chk_index* idx);
+
+/*
+ * Kernel timing with HIP events recorded on the index's stream around each
+ * named kernel ("sweep", "partition", "qprep", ...).  chk_profile_read
+ * synchronises and returns the number of launches and their summed duration.
+ */
+int chk_profile_enable(chk_index* idx, int on);
+int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

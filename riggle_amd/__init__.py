@@ -1,0 +1,203 @@
+"""Python binding of libchuckle_hip.so, the MI355X (gfx950) implementation of
+Chuckle's file-to-file search hot path.
+
+The compute runs in hand-written HIP kernels behind the C ABI declared in
+``include/chuckle_hip.h``; this module only marshals numpy arrays through
+ctypes.  There is no CPU fallback: if the library is missing, importing the
+handle types raises ``RuntimeError`` telling the caller to build it
+(``python -m riggle_amd.build``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libchuckle_hip.so")
+
+CHK_OK = 0
+_ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "I/O error",
+           5: "not found", 6: "internal error"}
+
+_lib = None
+
+
+class ChuckleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libchuckle_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m riggle_amd.build`"
+                               " (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    L.chk_version.restype = C.c_char_p
+    L.chk_last_error.restype = C.c_char_p
+    L.chk_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.chk_index_create.argtypes = [i32, u32, u32, u64, vp, vp, vp, vp, C.POINTER(vp)]
+    L.chk_index_free.argtypes = [vp]
+    L.chk_index_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)]
+    L.chk_query_create.argtypes = [vp, u32, u64, vp, vp, vp, vp, C.POINTER(vp)]
+    L.chk_query_free.argtypes = [vp]
+    L.chk_sweep.argtypes = [vp, vp, vp, vp]
+    L.chk_sweep_async.argtypes = [vp, vp, vp, vp]
+    L.chk_dev_alloc.argtypes = [vp, u64, C.POINTER(vp)]
+    L.chk_dev_free.argtypes = [vp, vp]
+    L.chk_dev_to_host.argtypes = [vp, vp, vp, u64]
+    L.chk_sync.argtypes = [vp]
+    L.chk_profile_enable.argtypes = [vp, i32]
+    L.chk_profile_read.argtypes = [vp, C.c_char_p, C.POINTER(u64), C.POINTER(C.c_double)]
+
+
+def _check(code):
+    if code != CHK_OK:
+        raise ChuckleError(code, lib().chk_last_error().decode())
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None and len(a) else None
+
+
+def version():
+    return lib().chk_version().decode()
+
+
+def device_count():
+    n = C.c_int()
+    _check(lib().chk_device_count(C.byref(n)))
+    return n.value
+
+
+class Index:
+    """HBM-resident DB index (per-shard start/end streams + file ids)."""
+
+    def __init__(self, shard, start, end, fid, nshards, nfiles, device=0):
+        shard, start, end, fid = _u32(shard), _u32(start), _u32(end), _u32(fid)
+        self._keep = (shard, start, end, fid)
+        h = C.c_void_p()
+        _check(lib().chk_index_create(device, nshards, nfiles, len(start), _ptr(shard),
+                                      _ptr(start), _ptr(end), _ptr(fid), C.byref(h)))
+        self._h = h
+        self._keep = None
+        self.nshards, self.nfiles, self.n = int(nshards), int(nfiles), len(start)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        a, b, n, by = C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(lib().chk_index_info(self._h, C.byref(a), C.byref(b), C.byref(n), C.byref(by)))
+        return {"nshards": a.value, "nfiles": b.value, "n": n.value, "device_bytes": by.value}
+
+    def query(self, shard, start, end, qfile=None, nfiles=1):
+        return Query(self, shard, start, end, qfile, nfiles)
+
+    def sweep(self, query):
+        """Exact (counts, bp) uint64 matrices [query files x DB files]."""
+        cnt = np.zeros((query.nfiles, self.nfiles), np.uint64)
+        bp = np.zeros((query.nfiles, self.nfiles), np.uint64)
+        _check(lib().chk_sweep(self._h, query.handle, cnt.ctypes.data_as(C.c_void_p),
+                               bp.ctypes.data_as(C.c_void_p)))
+        return cnt, bp
+
+    def sweep_async(self, query, d_counts, d_bp):
+        _check(lib().chk_sweep_async(self._h, query.handle, d_counts.ptr, d_bp.ptr))
+
+    def sync(self):
+        _check(lib().chk_sync(self._h))
+
+    def profile(self, on=True):
+        _check(lib().chk_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self, name):
+        n, ms = C.c_uint64(), C.c_double()
+        _check(lib().chk_profile_read(self._h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().chk_index_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Query:
+    """A batch of query files uploaded to the device."""
+
+    def __init__(self, index, shard, start, end, qfile=None, nfiles=1):
+        shard, start, end = _u32(shard), _u32(start), _u32(end)
+        qf = _u32(qfile) if qfile is not None else None
+        h = C.c_void_p()
+        _check(lib().chk_query_create(index.handle, nfiles, len(start), _ptr(shard), _ptr(start),
+                                      _ptr(end), _ptr(qf), C.byref(h)))
+        self._h = h
+        self._index = index
+        self.nfiles = int(nfiles)
+        self.n = len(start)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().chk_query_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    def __init__(self, index, nbytes):
+        p = C.c_void_p()
+        _check(lib().chk_dev_alloc(index.handle, nbytes, C.byref(p)))
+        self.ptr, self.nbytes, self._index = p, int(nbytes), index
+
+    def to_numpy(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().chk_dev_to_host(self._index.handle, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                     out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().chk_dev_free(self._index.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

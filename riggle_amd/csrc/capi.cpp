@@ -1,0 +1,252 @@
+// capi.cpp -- extern "C" entry points declared in include/chuckle_hip.h.
+// Each catches every exception, records the message for chk_last_error() and
+// returns a status code.
+#include <cstring>
+#include <new>
+
+#include "engine.hpp"
+
+struct chk_index {
+    chk::Index ix;
+};
+struct chk_query {
+    chk::Query q;
+};
+
+namespace chk {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace chk
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return CHK_OK;
+    } catch (const chk::Error& e) {
+        chk::set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        chk::set_last_error("host allocation failed");
+        return CHK_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        chk::set_last_error(e.what());
+        return CHK_ERR_INTERNAL;
+    } catch (...) {
+        chk::set_last_error("unknown error");
+        return CHK_ERR_INTERNAL;
+    }
+}
+
+void set_device(int device) { CHK_HIP(hipSetDevice(device)); }
+
+}  // namespace
+
+extern "C" {
+
+const char* chk_version(void) { return "chuckle-mi355x 0.1.0 (gfx950)"; }
+
+const char* chk_last_error(void) { return chk::g_last_error.c_str(); }
+
+int chk_device_count(int* count) {
+    return guarded([&] {
+        CHK_REQUIRE(count, CHK_ERR_INVALID, "count is NULL");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            n = 0;
+        }
+        *count = n;
+    });
+}
+
+int chk_index_create(int device, uint32_t nshards, uint32_t nfiles, uint64_t n,
+                     const uint32_t* shard, const uint32_t* start, const uint32_t* end,
+                     const uint32_t* fid, chk_index** out) {
+    return guarded([&] {
+        CHK_REQUIRE(out, CHK_ERR_INVALID, "out is NULL");
+        *out = nullptr;
+        CHK_REQUIRE(n == 0 || (shard && start && end && fid), CHK_ERR_INVALID,
+                    "interval arrays must be non-NULL");
+        CHK_REQUIRE(nfiles > 0, CHK_ERR_INVALID, "nfiles must be positive");
+        set_device(device);
+        auto* h = new chk_index();
+        try {
+            h->ix.device = device;
+            h->ix.nshards = nshards;
+            h->ix.nfiles = nfiles;
+            h->ix.n = n;
+            h->ix.ncu = chk::num_cus(device);
+            CHK_HIP(hipStreamCreateWithFlags(&h->ix.stream, hipStreamNonBlocking));
+            chk::index_build(h->ix, shard, start, end, fid);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int chk_index_free(chk_index* idx) {
+    return guarded([&] {
+        if (!idx) return;
+        set_device(idx->ix.device);
+        (void)hipStreamSynchronize(idx->ix.stream);
+        delete idx;
+    });
+}
+
+int chk_index_info(const chk_index* idx, uint32_t* nshards, uint32_t* nfiles, uint64_t* n,
+                   uint64_t* device_bytes) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "index is NULL");
+        if (nshards) *nshards = idx->ix.nshards;
+        if (nfiles) *nfiles = idx->ix.nfiles;
+        if (n) *n = idx->ix.n;
+        if (device_bytes) *device_bytes = idx->ix.device_bytes();
+    });
+}
+
+int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t* shard,
+                     const uint32_t* start, const uint32_t* end, const uint32_t* qfile,
+                     chk_query** out) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && out, CHK_ERR_INVALID, "index / out is NULL");
+        *out = nullptr;
+        CHK_REQUIRE(nfiles > 0, CHK_ERR_INVALID, "nfiles must be positive");
+        CHK_REQUIRE(n == 0 || (shard && start && end), CHK_ERR_INVALID,
+                    "interval arrays must be non-NULL");
+        CHK_REQUIRE(nfiles == 1 || qfile || n == 0, CHK_ERR_INVALID,
+                    "qfile is required when nfiles > 1");
+        set_device(idx->ix.device);
+        auto* h = new chk_query();
+        try {
+            h->q.idx = &idx->ix;
+            h->q.nfiles = nfiles;
+            h->q.n = n;
+            chk::query_upload(h->q, shard, start, end, qfile);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int chk_query_free(chk_query* q) {
+    return guarded([&] {
+        if (!q) return;
+        set_device(q->q.idx->device);
+        (void)hipStreamSynchronize(q->q.idx->stream);
+        delete q;
+    });
+}
+
+int chk_sweep_async(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_bp) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && q && d_counts && d_bp, CHK_ERR_INVALID, "NULL argument");
+        CHK_REQUIRE(q->q.idx == &idx->ix, CHK_ERR_INVALID, "query belongs to another index");
+        set_device(idx->ix.device);
+        chk::query_prepare(q->q);
+        chk::sweep_run(q->q, d_counts, d_bp);
+    });
+}
+
+int chk_sweep(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && q && counts && bp, CHK_ERR_INVALID, "NULL argument");
+        CHK_REQUIRE(q->q.idx == &idx->ix, CHK_ERR_INVALID, "query belongs to another index");
+        set_device(idx->ix.device);
+        size_t cells = (size_t)q->q.nfiles * idx->ix.nfiles;
+        chk::DevBuf<uint64_t> dc, db;
+        dc.reserve(cells);
+        db.reserve(cells);
+        chk::query_prepare(q->q);
+        chk::sweep_run(q->q, dc.p, db.p);
+        if (q->q.jobs_host.empty()) {
+            std::memset(counts, 0, cells * 8);
+            std::memset(bp, 0, cells * 8);
+            CHK_HIP(hipStreamSynchronize(idx->ix.stream));
+            return;
+        }
+        CHK_HIP(hipMemcpyAsync(counts, dc.p, cells * 8, hipMemcpyDeviceToHost, idx->ix.stream));
+        CHK_HIP(hipMemcpyAsync(bp, db.p, cells * 8, hipMemcpyDeviceToHost, idx->ix.stream));
+        CHK_HIP(hipStreamSynchronize(idx->ix.stream));
+    });
+}
+
+int chk_dev_alloc(chk_index* idx, uint64_t bytes, void** ptr) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && ptr, CHK_ERR_INVALID, "NULL argument");
+        set_device(idx->ix.device);
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw chk::Error(CHK_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+        *ptr = p;
+    });
+}
+
+int chk_dev_free(chk_index* idx, void* ptr) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");
+        set_device(idx->ix.device);
+        CHK_HIP(hipStreamSynchronize(idx->ix.stream));
+        if (ptr) CHK_HIP(hipFree(ptr));
+    });
+}
+
+int chk_dev_to_host(chk_index* idx, void* dst, const void* src, uint64_t bytes) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && dst && src, CHK_ERR_INVALID, "NULL argument");
+        set_device(idx->ix.device);
+        CHK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, idx->ix.stream));
+        CHK_HIP(hipStreamSynchronize(idx->ix.stream));
+    });
+}
+
+int chk_sync(chk_index* idx) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");
+        set_device(idx->ix.device);
+        CHK_HIP(hipStreamSynchronize(idx->ix.stream));
+    });
+}
+
+int chk_profile_enable(chk_index* idx, int on) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");
+        idx->ix.prof.on = on != 0;
+    });
+}
+
+int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, double* total_ms) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && name, CHK_ERR_INVALID, "NULL argument");
+        set_device(idx->ix.device);
+        CHK_HIP(hipStreamSynchronize(idx->ix.stream));
+        uint64_t cnt = 0;
+        double ms = 0.0;
+        auto it = idx->ix.prof.ev.find(name);
+        if (it != idx->ix.prof.ev.end()) {
+            for (auto& p : it->second) {
+                float t = 0.f;
+                CHK_HIP(hipEventElapsedTime(&t, p.first, p.second));
+                ms += t;
+                cnt++;
+                idx->ix.prof.pool.push_back(p.first);
+                idx->ix.prof.pool.push_back(p.second);
+            }
+            it->second.clear();
+        }
+        if (launches) *launches = cnt;
+        if (total_ms) *total_ms = ms;
+    });
+}
+
+}  // extern "C"

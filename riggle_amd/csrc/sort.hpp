@@ -1,0 +1,27 @@
+// sort.hpp -- device-wide primitives written for gfx950: stable LSD radix sort
+// of 64-bit keys (optional 32-bit payload) and exclusive scans.
+#pragma once
+
+#include "common.hpp"
+
+namespace chk {
+
+// Scratch reused across calls on one stream.
+struct SortScratch {
+    DevBuf<uint64_t> keys_alt;
+    DevBuf<uint32_t> vals_alt;
+    DevBuf<uint32_t> hist;
+    DevBuf<uint32_t> sums;
+};
+
+// Sorts keys[0..n) (and vals, when non-null) by bits [0, nbits) of the key,
+// stably.  Results end in keys / vals.
+void radix_sort_u64(uint64_t* keys, uint32_t* vals, size_t n, uint32_t nbits, SortScratch& s,
+                    hipStream_t st);
+
+// In-place exclusive scans of data[0..n); data[n] receives the total, so the
+// buffer must hold n+1 elements.
+void exclusive_scan_u32(uint32_t* data, size_t n, SortScratch& s, hipStream_t st);
+void exclusive_scan_i64(int64_t* data, size_t n, DevBuf<int64_t>& sums, hipStream_t st);
+
+}  // namespace chk

@@ -1,0 +1,154 @@
+"""Parity of the HIP merge-path sweep with the oracle (bit-exact u64).
+
+Mirrors the reference's sweep tests (src/sweep/query.rs:159-221), its seeded
+integration goldens (tests/integration.rs:57-115) and adds randomized cases
+against the oracle's brute-force and rank-identity forms.
+"""
+import numpy as np
+import pytest
+
+from tests.cases import flatten, integration_setup, random_files
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_sweep(chk, db, q, nshards, F, nq):
+    ix = chk.Index(db[0], db[1], db[2], db[3], nshards, F)
+    qq = ix.query(q[0], q[1], q[2], q[3] if nq > 1 else None, nq)
+    return ix.sweep(qq)
+
+
+def one(shard_start_end_sid):
+    return tuple(np.array(v, np.uint32) for v in zip(*shard_start_end_sid))
+
+
+def run_unit(chk, db, q, nq, F):
+    db = one(db) if db else tuple(np.zeros(0, np.uint32) for _ in range(4))
+    q = one(q) if q else tuple(np.zeros(0, np.uint32) for _ in range(4))
+    return gpu_sweep(chk, db, q, 1, F, nq)
+
+
+# --- src/sweep/query.rs:202-264 ------------------------------------------------
+
+def test_basic_overlap(chk):
+    c, bp = run_unit(chk, [(0, 50, 150, 0)], [(0, 40, 160, 0)], 1, 1)
+    assert c[0, 0] == 1 and bp[0, 0] == 100  # 1.0 bin
+
+
+def test_no_overlap(chk):
+    c, bp = run_unit(chk, [(0, 50, 100, 0)], [(0, 200, 300, 0)], 1, 1)
+    assert c[0, 0] == 0 and bp[0, 0] == 0
+
+
+def test_multiple_queries_and_db(chk):
+    c, bp = run_unit(chk, [(0, 25, 75, 0), (0, 75, 125, 1)], [(0, 0, 50, 0), (0, 50, 100, 1)], 2, 2)
+    assert c.tolist() == [[1, 0], [1, 1]]
+    assert bp.tolist() == [[25, 0], [25, 25]]
+
+
+def test_dead_zone(chk):
+    c, bp = run_unit(chk, [(0, 0, 50, 0), (0, 10_000, 10_050, 1)], [(0, 10_000, 10_100, 0)], 1, 2)
+    assert c.tolist() == [[0, 1]] and bp.tolist() == [[0, 50]]
+
+
+def test_touching_intervals_not_overlapping(chk):
+    c, bp = run_unit(chk, [(0, 0, 50, 0)], [(0, 50, 100, 0)], 1, 1)
+    assert c[0, 0] == 0 and bp[0, 0] == 0
+    c, bp = run_unit(chk, [(0, 50, 100, 0)], [(0, 0, 50, 0)], 1, 1)
+    assert c[0, 0] == 0 and bp[0, 0] == 0
+
+
+def test_empty_db(chk):
+    c, bp = run_unit(chk, [], [(0, 0, 100, 0)], 1, 1)
+    assert c[0, 0] == 0 and bp[0, 0] == 0
+
+
+def test_empty_query(chk):
+    c, bp = run_unit(chk, [(0, 0, 100, 0)], [], 1, 1)
+    assert c[0, 0] == 0 and bp[0, 0] == 0
+
+
+def test_shard_not_in_db(chk):
+    # src/tasks/query/mod.rs:393-412: a query shard absent from the DB gives nothing
+    db = one([(0, 100, 200, 0)])
+    q = one([(7, 100, 200, 0)])  # id >= nshards: shard unknown to the DB
+    c, bp = gpu_sweep(chk, db, q, 1, 1, 1)
+    assert c[0, 0] == 0 and bp[0, 0] == 0
+
+
+# --- tests/integration.rs goldens ---------------------------------------------
+
+def test_golden_total_hits_and_bp(chk, orc):
+    db, q = integration_setup(orc, 10, 2000, 1, 1000, 50_000_000)
+    c, bp = gpu_sweep(chk, db, q, 24, 10, 1)
+    assert int(c.sum()) == 158 and int(bp.sum()) == 413_652
+
+
+def test_golden_counts_nonzero_pairs(chk, orc):
+    db, q = integration_setup(orc, 5, 1000, 2, 500, 30_000_000)
+    c, bp = gpu_sweep(chk, db, q, 24, 5, 2)
+    assert int((c > 0).sum()) == 10 and int(c.sum()) == 58
+    oc, obp = orc.sweep_bruteforce(db, q, 5, 2)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+# --- randomized parity ----------------------------------------------------------
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_vs_bruteforce(chk, orc, seed):
+    rng = np.random.default_rng(seed)
+    F, nq, S = 7, 3, 4
+    db = flatten(random_files(rng, F, S, (0, 400), 20_000, 2_000))
+    q = flatten(random_files(rng, nq, S + 1, (0, 300), 20_000, 3_000))  # shard S unknown to DB
+    c, bp = gpu_sweep(chk, db, q, S, F, nq)
+    oc, obp = orc.sweep_bruteforce(db, q, F, nq)
+    assert (c.astype(np.int64) == oc).all()
+    assert (bp.astype(np.int64) == obp).all()
+
+
+def test_ties_and_nesting(chk, orc):
+    # identical coordinates, zero-gap touching, nested, 1-bp intervals
+    rng = np.random.default_rng(9)
+    n = 3000
+    s = rng.integers(0, 60, n).astype(np.uint32) * 10
+    e = s + rng.integers(1, 8, n).astype(np.uint32) * 10
+    e[::7] = s[::7] + 1
+    db = (np.zeros(n, np.uint32), s, e, rng.integers(0, 5, n).astype(np.uint32))
+    qs = rng.integers(0, 60, 500).astype(np.uint32) * 10
+    q = (np.zeros(500, np.uint32), qs, qs + rng.integers(1, 6, 500).astype(np.uint32) * 10,
+         rng.integers(0, 2, 500).astype(np.uint32))
+    c, bp = gpu_sweep(chk, db, q, 1, 5, 2)
+    oc, obp = orc.sweep_bruteforce(db, q, 5, 2)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+def test_large_vs_ranks(chk, orc):
+    rng = np.random.default_rng(11)
+    F, nq, S = 40, 4, 6
+    db = flatten(random_files(rng, F, S, (1000, 12_000), 5_000_000, 20_000))
+    q = flatten(random_files(rng, nq, S, (10, 30_000), 5_000_000, 20_000))
+    c, bp = gpu_sweep(chk, db, q, S, F, nq)
+    oc, obp = orc.sweep_ranks(db, q, F, nq, S)
+    assert (c.astype(np.int64) == oc).all()
+    assert (bp.astype(np.int64) == obp).all()
+
+
+def test_query_dense_tiles(chk, orc):
+    # a query much larger than the DB: most tiles hold only query events
+    rng = np.random.default_rng(5)
+    db = flatten(random_files(rng, 3, 2, 50, 1_000_000, 5_000))
+    q = flatten(random_files(rng, 1, 2, 200_000, 1_000_000, 300))
+    c, bp = gpu_sweep(chk, db, q, 2, 3, 1)
+    oc, obp = orc.sweep_ranks(db, q, 3, 1, 2)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+def test_repeat_is_deterministic(chk, orc):
+    rng = np.random.default_rng(3)
+    db = flatten(random_files(rng, 20, 3, 5000, 2_000_000, 5_000))
+    q = flatten(random_files(rng, 2, 3, 4000, 2_000_000, 5_000))
+    ix = chk.Index(*db, 3, 20)
+    qq = ix.query(q[0], q[1], q[2], q[3], 2)
+    a = ix.sweep(qq)
+    b = ix.sweep(qq)
+    assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
