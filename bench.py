@@ -72,6 +72,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tune", action="append", default=[],
+                    help="launch-shape knob key=value (see chk_tune), may repeat")
+    ap.add_argument("--calibrate-bytes", type=int, default=0,
+                    help="also run the plain copy kernel over this many bytes (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -93,6 +97,9 @@ def main():
     ix = riggle_amd.Index(sh, s, e, fid, nshards, F)
     t_index = time.time() - t
     log(f"index built in {t_index:.2f}s: {ix.info()}")
+    for kv in args.tune:
+        k, v = kv.split("=")
+        ix.tune(k, int(v))
     qq = ix.query(qsh, qs, qe)
     dc, db_ = ix.alloc(8 * F), ix.alloc(8 * F)
 
@@ -115,6 +122,15 @@ def main():
     counts = dc.to_numpy(np.uint64, F)
     bp = db_.to_numpy(np.uint64, F)
     log(f"timed {args.steps} steps: {ms_step:.3f} ms/step; kernels {kern}")
+    calib = None
+    if args.calibrate_bytes > 0:
+        ix.profile(True)
+        ix.calibrate_copy(args.calibrate_bytes)
+        n_c, ms_c = ix.profile_read("calib_copy")
+        ix.profile(False)
+        calib = {"bytes": args.calibrate_bytes, "ms": ms_c,
+                 "gbs_rw": 2 * args.calibrate_bytes / (ms_c * 1e-3) / 1e9}
+        log(f"calibration copy: {calib}")
 
     swept = (Q + D) / (ms_step * 1e-3)
     n_sweep, ms_sweep = kern["sweep"]
@@ -136,8 +152,11 @@ def main():
                      "alg_bytes_per_launch": alg_bytes},
         "kernels_ms_per_step": {k: (v[1] / max(v[0], 1)) for k, v in kern.items()},
         "index_build_s": t_index,
+        "tuning": args.tune,
         "result_check": {"total_pairs": int(counts.sum()), "total_bp": int(bp.sum())},
     }
+    if calib:
+        rec["calibration_copy"] = calib
 
     if not args.no_cpu:
         rec_cpu, cnt_cpu, sel = cpu_baseline((sh, s, e, fid), (qsh, qs, qe), nshards, F)

@@ -105,6 +105,19 @@ int chk_sync(chk_index* idx);
 int chk_profile_enable(chk_index* idx, int on);
 int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, double* total_ms);
 
+/*
+ * Launch-shape knobs of the sweep, set by the autotuner or by tests
+ * (value 0 restores the automatic choice):
+ *   "sweep.blocks_per_cu"   resident blocks per CU the persistent grid targets
+ *   "sweep.tiles_per_block" contiguous merge-path tiles per block
+ *   "sweep.part_window"     partition entries staged in LDS at a time (2..1024)
+ */
+int chk_tune(chk_index* idx, const char* key, int64_t value);
+
+/* Launch a plain device copy kernel ("calib_copy") that reads and writes
+ * `bytes` bytes once; used to scale hardware byte counters. */
+int chk_calibrate_copy(chk_index* idx, uint64_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libchuckle_hip.so")
+LIB_PATH = os.environ.get("CHUCKLE_HIP_LIB") or os.path.join(HERE, "libchuckle_hip.so")
 
 CHK_OK = 0
 _ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "I/O error",
@@ -61,6 +61,8 @@ def _declare(L):
     L.chk_sync.argtypes = [vp]
     L.chk_profile_enable.argtypes = [vp, i32]
     L.chk_profile_read.argtypes = [vp, C.c_char_p, C.POINTER(u64), C.POINTER(C.c_double)]
+    L.chk_calibrate_copy.argtypes = [vp, u64]
+    L.chk_tune.argtypes = [vp, C.c_char_p, C.c_int64]
 
 
 def _check(code):
@@ -132,6 +134,12 @@ class Index:
         n, ms = C.c_uint64(), C.c_double()
         _check(lib().chk_profile_read(self._h, name.encode(), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def tune(self, key, value):
+        _check(lib().chk_tune(self._h, key.encode(), int(value)))
+
+    def calibrate_copy(self, nbytes):
+        _check(lib().chk_calibrate_copy(self._h, nbytes))
 
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)

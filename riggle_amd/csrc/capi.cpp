@@ -249,4 +249,25 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
     });
 }
 
+int chk_tune(chk_index* idx, const char* key, int64_t value) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && key, CHK_ERR_INVALID, "NULL argument");
+        CHK_REQUIRE(value >= 0 && value <= 0xFFFFFFFFll, CHK_ERR_INVALID, "tuning value out of range");
+        std::string k(key);
+        chk::Tuning& t = idx->ix.tune;
+        if (k == "sweep.blocks_per_cu") t.blocks_per_cu = (uint32_t)value;
+        else if (k == "sweep.tiles_per_block") t.tiles_per_block = (uint32_t)value;
+        else if (k == "sweep.part_window") t.part_window = (uint32_t)value;
+        else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
+    });
+}
+
+int chk_calibrate_copy(chk_index* idx, uint64_t bytes) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");
+        set_device(idx->ix.device);
+        chk::calibrate_copy(idx->ix, bytes);
+    });
+}
+
 }  // extern "C"

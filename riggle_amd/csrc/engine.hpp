@@ -49,8 +49,16 @@ struct Profiler {
     }
 };
 
+// Launch-shape knobs of the sweep (0 = choose automatically).
+struct Tuning {
+    uint32_t blocks_per_cu = 0;    // resident blocks per CU the grid is sized for
+    uint32_t tiles_per_block = 0;  // contiguous tiles per block
+    uint32_t part_window = 0;      // partition entries per LDS window (<= 1024)
+};
+
 // Per-shard sorted streams of the DB (structure of arrays in HBM).
 struct Index {
+    Tuning tune;
     int device = 0;
     hipStream_t stream = nullptr;
     int ncu = 256;
@@ -58,14 +66,17 @@ struct Index {
     uint64_t n = 0;
     std::vector<uint64_t> shard_off;  // nshards+1, offsets into the streams
     DevBuf<int32_t> s_coord, e_coord;  // starts / ends sorted per shard
-    DevBuf<uint32_t> s_fid, e_fid;     // file id in the same order
+    DevBuf<uint32_t> s_fid, e_fid;     // file id in the same order (u32 form)
+    DevBuf<uint16_t> s_fid16, e_fid16; // ... u16 form, used when nfiles <= 65536
+    bool fid16 = false;
     SortScratch scratch;
     Profiler prof;
     ~Index() {
         if (stream) (void)hipStreamDestroy(stream);
     }
     uint64_t device_bytes() const {
-        return s_coord.bytes() + e_coord.bytes() + s_fid.bytes() + e_fid.bytes();
+        return s_coord.bytes() + e_coord.bytes() + s_fid.bytes() + e_fid.bytes() +
+               s_fid16.bytes() + e_fid16.bytes();
     }
 };
 
@@ -90,6 +101,7 @@ struct Query {
     DevBuf<uint32_t> shard, start, end, qfile;  // uploaded intervals
     // prepared form (rebuilt by every sweep)
     DevBuf<uint64_t> keys;     // 2n event keys, sorted
+    DevBuf<uint32_t> key32;    // their low 32 bits (2*coord + is_start)
     DevBuf<uint32_t> rs;       // 2n+1 exclusive prefix: # start events
     DevBuf<int64_t> w;         // 2n+1 exclusive prefix: sum(start) - sum(end)
     DevBuf<int64_t> w_sums;
@@ -108,5 +120,7 @@ void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const 
 void query_prepare(Query& q);
 // Merge-path partition + sweep into device count / bp matrices (zeroed here).
 void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp);
+// Plain copy of `bytes` (reads and writes that many bytes once).
+void calibrate_copy(Index& ix, uint64_t bytes);
 
 }  // namespace chk

@@ -32,9 +32,9 @@ __global__ void split_index_keys(const uint64_t* keys, size_t n, int32_t* coord)
     if (i < n) coord[i] = (int32_t)(keys[i] & 0x7FFFFFFFull);
 }
 
-__global__ void iota_u32(uint32_t* v, size_t n, const uint32_t* src) {
+__global__ void narrow_u16(const uint32_t* src, size_t n, uint16_t* dst) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) v[i] = src[i];
+    if (i < n) dst[i] = (uint16_t)src[i];
 }
 
 // Two events per query interval.  Dropped intervals (shard not in the DB)
@@ -60,12 +60,13 @@ __global__ void make_event_keys(const uint32_t* shard, const uint32_t* start, co
 // be prefix-summed.
 __global__ void segment_and_values(const uint64_t* keys, size_t m, uint32_t nshards,
                                    int shard_bits, uint32_t nseg, uint32_t* seg, uint32_t* rs,
-                                   int64_t* w) {
+                                   int64_t* w, uint32_t* key32) {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
     uint64_t key = keys[k];
     bool valid = key != ~0ull;
     uint32_t lo = (uint32_t)key;
+    key32[k] = lo;
     rs[k] = valid ? (lo & 1u) : 0u;
     int64_t c = (int64_t)(lo >> 1);
     w[k] = valid ? ((lo & 1u) ? c : -c) : 0;
@@ -118,6 +119,17 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
         hipLaunchKernelGGL(split_index_keys, dim3(nblk(n)), dim3(NT), 0, st, keys.p, n, out_coord);
         CHK_HIP(hipGetLastError());
     }
+    if (ix.nfiles <= 65536) {  // 2-byte file ids: 12 bytes per interval over both streams
+        ix.fid16 = true;
+        ix.s_fid16.reserve(n);
+        ix.e_fid16.reserve(n);
+        hipLaunchKernelGGL(narrow_u16, dim3(nblk(n)), dim3(NT), 0, st, ix.s_fid.p, n, ix.s_fid16.p);
+        hipLaunchKernelGGL(narrow_u16, dim3(nblk(n)), dim3(NT), 0, st, ix.e_fid.p, n, ix.e_fid16.p);
+        CHK_HIP(hipGetLastError());
+        CHK_HIP(hipStreamSynchronize(st));
+        ix.s_fid.release();
+        ix.e_fid.release();
+    }
     CHK_HIP(hipStreamSynchronize(st));
     ix.scratch.keys_alt.release();
     ix.scratch.vals_alt.release();
@@ -157,11 +169,13 @@ void query_prepare(Query& q) {
     const int sb = shard_bits ? shard_bits : 1;
     const uint32_t nseg = q.nfiles * ix.nshards;
     q.keys.reserve(m);
+    q.key32.reserve(m + 1);
     q.rs.reserve(m + 1);
     q.w.reserve(m + 1);
     q.seg.reserve(2 * (size_t)nseg);
     hipEvent_t t0 = ix.prof.begin(st);
     CHK_HIP(hipMemsetAsync(q.seg.p, 0, 2 * (size_t)nseg * 4, st));
+    CHK_HIP(hipMemsetAsync(q.key32.p + m, 0xFF, sizeof(uint32_t), st));  // sentinel
     if (m) {
         hipLaunchKernelGGL(make_event_keys, dim3(nblk(q.n)), dim3(NT), 0, st, q.shard.p, q.start.p,
                            q.end.p, q.nfiles > 1 ? q.qfile.p : nullptr, (size_t)q.n, ix.nshards, sb,
@@ -169,7 +183,7 @@ void query_prepare(Query& q) {
         uint32_t nbits = 32 + sb + bits_for(q.nfiles);
         radix_sort_u64(q.keys.p, nullptr, m, nbits, ix.scratch, st);
         hipLaunchKernelGGL(segment_and_values, dim3(nblk(m)), dim3(NT), 0, st, q.keys.p, m,
-                           ix.nshards, sb, nseg, q.seg.p, q.rs.p, q.w.p);
+                           ix.nshards, sb, nseg, q.seg.p, q.rs.p, q.w.p, q.key32.p);
         CHK_HIP(hipGetLastError());
     }
     exclusive_scan_u32(q.rs.p, m, ix.scratch, st);

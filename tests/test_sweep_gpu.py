@@ -152,3 +152,27 @@ def test_repeat_is_deterministic(chk, orc):
     a = ix.sweep(qq)
     b = ix.sweep(qq)
     assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
+
+
+@pytest.mark.parametrize("tpb,pwin,bpc", [(3, 4, 1), (40, 2, 2), (1, 2, 0)])
+def test_launch_shapes_agree(chk, orc, tpb, pwin, bpc):
+    # tiny partition windows and tile ranges force the LDS window refills,
+    # job-span changes and query-file switches inside one block
+    rng = np.random.default_rng(21)
+    F, nq, S = 12, 5, 7
+    db = flatten(random_files(rng, F, S, (2000, 9000), 3_000_000, 8_000))
+    q = flatten(random_files(rng, nq, S, (100, 20_000), 3_000_000, 8_000))
+    ix = chk.Index(*db, S, F)
+    ix.tune("sweep.tiles_per_block", tpb)
+    ix.tune("sweep.part_window", pwin)
+    ix.tune("sweep.blocks_per_cu", bpc)
+    c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3], nq))
+    oc, obp = orc.sweep_ranks(db, q, F, nq, S)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+def test_tune_rejects_unknown_key(chk):
+    ix = chk.Index(np.zeros(1, np.uint32), np.zeros(1, np.uint32), np.ones(1, np.uint32),
+                   np.zeros(1, np.uint32), 1, 1)
+    with pytest.raises(chk.ChuckleError):
+        ix.tune("sweep.nonsense", 1)
