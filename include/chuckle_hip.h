@@ -91,6 +91,41 @@ int chk_sweep(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp);
  * without a final synchronisation. */
 int chk_sweep_async(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_bp);
 
+/*
+ * Genome table used by the statistics (src/fourier/genome.rs:7-34 is the
+ * reference's fixed hg38 table): shard_size[s] = length in bp of shard s, or 0
+ * for a shard outside the table (no moments, no p-value terms), as the
+ * reference ignores chromosomes absent from its table.  n_bins = ceil(size/100).
+ * Must be set before queries whose p-values are wanted are created.
+ */
+int chk_index_set_genome(chk_index* idx, const uint64_t* shard_size);
+
+/*
+ * Install compact moment tables (the reference's momentmap store,
+ * src/io/momentstore.rs:20-106), one per (DB file f, shard s):
+ * slot_off[f*nshards+s] = first entry in mean/var, or -1 when the file stores
+ * no table for the shard; slot_len = entries in the slot; entry i of a slot is
+ * block size L with compact_index(L) == i (src/fourier/moments.rs:58-66).
+ */
+int chk_moments_upload(chk_index* idx, const int64_t* slot_off, const uint32_t* slot_len,
+                       uint64_t total, const double* mean, const double* var);
+
+/*
+ * Analytic p-values (replaces compute_analytic_pvalues +
+ * compute_analytic_stats, src/tasks/query/pvalues.rs:22-77,
+ * src/fourier/stats.rs:34-126) for every cell with count > 0 of a sweep
+ * result.  Per cell: p, llr and flags (bit 0: a record is emitted -- count > 0,
+ * the DB file has moments and shares a chromosome with the query; bit 1: the
+ * LLR exists).  Cells without bit 0 carry no record (the reference's absent
+ * entry, p = 1).  Device arrays; enqueued without final synchronisation.
+ */
+int chk_pvalues_async(chk_index* idx, chk_query* q, const uint64_t* d_counts, const uint64_t* d_bp,
+                      double* d_p, double* d_llr, uint8_t* d_flags);
+
+/* Sweep + p-values into host arrays of nfiles(query) x nfiles(index); blocks. */
+int chk_query_stats(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp, double* p,
+                    double* llr, uint8_t* flags);
+
 /* Device memory helpers for callers without a device allocator. */
 int chk_dev_alloc(chk_index* idx, uint64_t bytes, void** ptr);
 int chk_dev_free(chk_index* idx, void* ptr);

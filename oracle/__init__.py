@@ -95,6 +95,8 @@ def _declare(L):
     L.orc_autocorr_fft.argtypes = [f64p, sz, f64p]
     L.orc_moments_reference.argtypes = [f32p, C.c_uint32, u32p, sz, f64p, f64p]
     L.orc_moments_exact.argtypes = [i64p, C.c_uint32, u32p, sz, f64p, f64p, i64p, f64p]
+    L.orc_moments_exact_fft.argtypes = [i64p, C.c_uint32, u32p, sz, f64p, f64p]
+    L.orc_moments_exact_fft.restype = C.c_int
     L.orc_erfc_as.argtypes = [C.c_double]
     L.orc_erfc_as.restype = C.c_double
     L.orc_nb_llr.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
@@ -263,12 +265,27 @@ def moments_exact(G, ls):
     return m, v, s1, s2
 
 
+def moments_exact_fft(G, ls):
+    """Exact (mean, var) via integer autocorrelation; None if the FFT rounding
+    check fails."""
+    ls = _u32(ls)
+    m = np.zeros(len(ls), np.float64)
+    v = np.zeros(len(ls), np.float64)
+    ok = lib().orc_moments_exact_fft(np.ascontiguousarray(G, np.int64), len(G), ls, len(ls), m, v)
+    return (m, v) if ok else None
+
+
 def chrom_moments_exact(start, end, n_bins):
     """Exact compact table (mean, var) for one DB chromosome."""
     ls = compact_ls(n_bins)
     if len(ls) == 0:
         return ls, np.zeros(0), np.zeros(0)
-    m, v, _, _ = moments_exact(depth_exact(start, end, n_bins), ls)
+    G = depth_exact(start, end, n_bins)
+    if n_bins > 4096:
+        r = moments_exact_fft(G, ls)
+        if r is not None:
+            return (ls,) + r
+    m, v, _, _ = moments_exact(G, ls)
     return ls, m, v
 
 

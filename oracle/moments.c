@@ -245,3 +245,74 @@ void orc_moments_exact(const int64_t* G, uint32_t n, const uint32_t* ls, size_t 
     }
     free(P);
 }
+
+/* Exact moments through the autocorrelation identity (valid for any integer
+ * signal, here H = G - c with c = round(mean G)):
+ *   S2_H(L) = L*A[0] + 2*sum_{m=1}^{L-1} (L-m) A[m]
+ *             - sum_{k=1}^{L-1} PH[k]^2 - sum_{k=1}^{L-1} (PH[n] - PH[n-k])^2,
+ * A[m] = sum_k H[k] H[k+m], PH = prefix sums of H.  A comes from this file's
+ * FP64 FFT rounded to integers; the rounding is accepted only when every value
+ * is within 0.25 of an integer and A[0] equals sum H^2 computed exactly, else
+ * the function returns 0 and the caller falls back to orc_moments_exact.
+ * Everything after A is exact 128-bit integer arithmetic. */
+int orc_moments_exact_fft(const int64_t* G, uint32_t n, const uint32_t* ls, size_t nl,
+                          double* mean, double* var) {
+    if (n == 0) return 1;
+    __int128 tot = 0;
+    for (uint32_t j = 0; j < n; j++) tot += G[j];
+    double md = (double)tot / (double)n;
+    int64_t c = (int64_t)llround(md);
+    double* h = (double*)malloc((size_t)n * sizeof(double));
+    __int128 a0 = 0;
+    for (uint32_t j = 0; j < n; j++) {
+        int64_t v = G[j] - c;
+        h[j] = (double)v;
+        a0 += (__int128)v * v;
+    }
+    double* r = (double*)malloc((size_t)n * sizeof(double));
+    orc_autocorr_fft(h, n, r);
+    int64_t* A = (int64_t*)malloc((size_t)n * sizeof(int64_t));
+    int ok = 1;
+    for (uint32_t m = 0; m < n; m++) {
+        double rr = nearbyint(r[m]);
+        if (fabs(r[m] - rr) > 0.25) ok = 0;
+        A[m] = (int64_t)rr;
+    }
+    if ((__int128)A[0] != a0) ok = 0;
+    if (!ok) {
+        free(h); free(r); free(A);
+        return 0;
+    }
+    int64_t* P = (int64_t*)malloc(((size_t)n + 1) * sizeof(int64_t));
+    P[0] = 0;
+    for (uint32_t j = 0; j < n; j++) P[j + 1] = P[j] + G[j];
+    __int128* PP = (__int128*)malloc(((size_t)n + 2) * sizeof(__int128));
+    PP[0] = 0;
+    for (uint32_t j = 0; j <= n; j++) PP[j + 1] = PP[j] + P[j];
+    /* running prefix sums over m / k, advanced to each L in ascending order */
+    __int128 C1 = 0, C2 = 0, E1 = 0, E2 = 0;
+    size_t cur = 1; /* sums cover indices 1..cur-1 */
+    __int128 PHn = (__int128)P[n] - (__int128)c * n;
+    for (size_t q = 0; q < nl; q++) {
+        size_t L = ls[q];
+        while (cur < L) {
+            C1 += A[cur];
+            C2 += (__int128)cur * A[cur];
+            __int128 phk = (__int128)P[cur] - (__int128)c * cur;
+            __int128 phm = PHn - ((__int128)P[n - cur] - (__int128)c * (n - cur));
+            E1 += phk * phk;
+            E2 += phm * phm;
+            cur++;
+        }
+        size_t nw = (size_t)n - L + 1;
+        __int128 T = (__int128)L * A[0] + 2 * ((__int128)L * C1 - C2);
+        __int128 S2 = T - E1 - E2;
+        __int128 S1G = PP[n + 1] - PP[L] - PP[n - L + 1];
+        __int128 S1H = S1G - (__int128)c * L * nw;
+        __int128 num = S2 * (__int128)nw - S1H * S1H;
+        mean[q] = (double)S1G / (100.0 * (double)nw);
+        var[q] = (double)num / (1e4 * (double)nw * (double)nw);
+    }
+    free(h); free(r); free(A); free(P); free(PP);
+    return 1;
+}

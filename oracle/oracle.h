@@ -113,6 +113,8 @@ void orc_moments_reference(const float* g, uint32_t n_bins,
 void orc_moments_exact(const int64_t* G, uint32_t n_bins, const uint32_t* ls,
                        size_t nl, double* mean, double* var, int64_t* s1,
                        double* s2);
+int orc_moments_exact_fft(const int64_t* G, uint32_t n_bins, const uint32_t* ls,
+                          size_t nl, double* mean, double* var);
 
 /* ---------------------------------------------------------------- stats.c */
 double orc_erfc_as(double x);

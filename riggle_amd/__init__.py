@@ -63,6 +63,10 @@ def _declare(L):
     L.chk_profile_read.argtypes = [vp, C.c_char_p, C.POINTER(u64), C.POINTER(C.c_double)]
     L.chk_calibrate_copy.argtypes = [vp, u64]
     L.chk_tune.argtypes = [vp, C.c_char_p, C.c_int64]
+    L.chk_index_set_genome.argtypes = [vp, vp]
+    L.chk_moments_upload.argtypes = [vp, vp, vp, u64, vp, vp]
+    L.chk_pvalues_async.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.chk_query_stats.argtypes = [vp, vp, vp, vp, vp, vp, vp]
 
 
 def _check(code):
@@ -134,6 +138,37 @@ class Index:
         n, ms = C.c_uint64(), C.c_double()
         _check(lib().chk_profile_read(self._h, name.encode(), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def set_genome(self, shard_size):
+        """Per-shard length in bp (0 = not in the genome table)."""
+        a = np.ascontiguousarray(shard_size, np.uint64)
+        assert len(a) == self.nshards
+        _check(lib().chk_index_set_genome(self._h, a.ctypes.data_as(C.c_void_p)))
+
+    def upload_moments(self, slot_off, slot_len, mean, var):
+        so = np.ascontiguousarray(slot_off, np.int64)
+        sl = np.ascontiguousarray(slot_len, np.uint32)
+        m = np.ascontiguousarray(mean, np.float64)
+        v = np.ascontiguousarray(var, np.float64)
+        assert len(so) == self.nfiles * self.nshards == len(sl)
+        _check(lib().chk_moments_upload(self._h, so.ctypes.data_as(C.c_void_p),
+                                        sl.ctypes.data_as(C.c_void_p), len(m),
+                                        m.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p)))
+
+    def stats(self, query):
+        """Sweep + p-values: dict of [query files x DB files] arrays."""
+        sh = (query.nfiles, self.nfiles)
+        out = {"counts": np.zeros(sh, np.uint64), "bp": np.zeros(sh, np.uint64),
+               "p": np.zeros(sh, np.float64), "llr": np.zeros(sh, np.float64),
+               "flags": np.zeros(sh, np.uint8)}
+        _check(lib().chk_query_stats(self._h, query.handle,
+                                     *[out[k].ctypes.data_as(C.c_void_p)
+                                       for k in ("counts", "bp", "p", "llr", "flags")]))
+        return out
+
+    def pvalues_async(self, query, d_counts, d_bp, d_p, d_llr, d_flags):
+        _check(lib().chk_pvalues_async(self._h, query.handle, d_counts.ptr, d_bp.ptr, d_p.ptr,
+                                       d_llr.ptr, d_flags.ptr))
 
     def tune(self, key, value):
         _check(lib().chk_tune(self._h, key.encode(), int(value)))

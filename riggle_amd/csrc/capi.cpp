@@ -128,6 +128,7 @@ int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t
             h->q.nfiles = nfiles;
             h->q.n = n;
             chk::query_upload(h->q, shard, start, end, qfile);
+            chk::query_groups(h->q, shard, start, end, qfile);
         } catch (...) {
             delete h;
             throw;
@@ -166,15 +167,74 @@ int chk_sweep(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp) {
         db.reserve(cells);
         chk::query_prepare(q->q);
         chk::sweep_run(q->q, dc.p, db.p);
-        if (q->q.jobs_host.empty()) {
-            std::memset(counts, 0, cells * 8);
-            std::memset(bp, 0, cells * 8);
-            CHK_HIP(hipStreamSynchronize(idx->ix.stream));
-            return;
-        }
         CHK_HIP(hipMemcpyAsync(counts, dc.p, cells * 8, hipMemcpyDeviceToHost, idx->ix.stream));
         CHK_HIP(hipMemcpyAsync(bp, db.p, cells * 8, hipMemcpyDeviceToHost, idx->ix.stream));
         CHK_HIP(hipStreamSynchronize(idx->ix.stream));
+    });
+}
+
+int chk_index_set_genome(chk_index* idx, const uint64_t* shard_size) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && shard_size, CHK_ERR_INVALID, "NULL argument");
+        set_device(idx->ix.device);
+        chk::Moments& M = idx->ix.mom;
+        M.nbins.resize(idx->ix.nshards);
+        for (uint32_t s = 0; s < idx->ix.nshards; s++) {
+            CHK_REQUIRE(shard_size[s] <= 0xFFFFFFFFull, CHK_ERR_INVALID, "shard size exceeds u32");
+            M.nbins[s] = (uint32_t)((shard_size[s] + 99) / 100);
+        }
+        M.d_nbins.reserve(idx->ix.nshards);
+        CHK_HIP(hipMemcpy(M.d_nbins.p, M.nbins.data(), idx->ix.nshards * 4, hipMemcpyHostToDevice));
+        M.genome = true;
+    });
+}
+
+int chk_moments_upload(chk_index* idx, const int64_t* slot_off, const uint32_t* slot_len,
+                       uint64_t total, const double* mean, const double* var) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && slot_off && slot_len && (total == 0 || (mean && var)), CHK_ERR_INVALID,
+                    "NULL argument");
+        set_device(idx->ix.device);
+        chk::moments_upload(idx->ix, slot_off, slot_len, total, mean, var);
+    });
+}
+
+int chk_pvalues_async(chk_index* idx, chk_query* q, const uint64_t* d_counts, const uint64_t* d_bp,
+                      double* d_p, double* d_llr, uint8_t* d_flags) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && q && d_counts && d_bp && d_p && d_llr && d_flags, CHK_ERR_INVALID,
+                    "NULL argument");
+        CHK_REQUIRE(q->q.idx == &idx->ix, CHK_ERR_INVALID, "query belongs to another index");
+        set_device(idx->ix.device);
+        chk::pvalues_run(q->q, d_counts, d_bp, d_p, d_llr, d_flags);
+    });
+}
+
+int chk_query_stats(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp, double* p,
+                    double* llr, uint8_t* flags) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && q && counts && bp && p && llr && flags, CHK_ERR_INVALID, "NULL argument");
+        CHK_REQUIRE(q->q.idx == &idx->ix, CHK_ERR_INVALID, "query belongs to another index");
+        set_device(idx->ix.device);
+        hipStream_t st = idx->ix.stream;
+        size_t cells = (size_t)q->q.nfiles * idx->ix.nfiles;
+        chk::DevBuf<uint64_t> dc, db;
+        chk::DevBuf<double> dp, dl;
+        chk::DevBuf<uint8_t> df;
+        dc.reserve(cells);
+        db.reserve(cells);
+        dp.reserve(cells);
+        dl.reserve(cells);
+        df.reserve(cells);
+        chk::query_prepare(q->q);
+        chk::sweep_run(q->q, dc.p, db.p);
+        chk::pvalues_run(q->q, dc.p, db.p, dp.p, dl.p, df.p);
+        CHK_HIP(hipMemcpyAsync(counts, dc.p, cells * 8, hipMemcpyDeviceToHost, st));
+        CHK_HIP(hipMemcpyAsync(bp, db.p, cells * 8, hipMemcpyDeviceToHost, st));
+        CHK_HIP(hipMemcpyAsync(p, dp.p, cells * 8, hipMemcpyDeviceToHost, st));
+        CHK_HIP(hipMemcpyAsync(llr, dl.p, cells * 8, hipMemcpyDeviceToHost, st));
+        CHK_HIP(hipMemcpyAsync(flags, df.p, cells, hipMemcpyDeviceToHost, st));
+        CHK_HIP(hipStreamSynchronize(st));
     });
 }
 

@@ -56,9 +56,25 @@ struct Tuning {
     uint32_t part_window = 0;      // partition entries per LDS window (<= 1024)
 };
 
+// Compact sliding-window moment tables of the DB files (FP64, device).
+struct Moments {
+    bool genome = false;
+    std::vector<uint32_t> nbins;       // per shard; 0 = shard not in the genome table
+    DevBuf<uint32_t> d_nbins;
+    bool present = false;
+    std::vector<int64_t> slot_off;     // [nfiles * nshards]: first entry or -1
+    std::vector<uint32_t> slot_len;    // entries per slot
+    DevBuf<int64_t> d_slot_off;
+    DevBuf<uint32_t> d_slot_len;
+    DevBuf<uint8_t> d_has_entry;       // [nfiles]: file has a stored entry
+    DevBuf<double> mean, var;
+    uint64_t total = 0;
+};
+
 // Per-shard sorted streams of the DB (structure of arrays in HBM).
 struct Index {
     Tuning tune;
+    Moments mom;
     int device = 0;
     hipStream_t stream = nullptr;
     int ncu = 256;
@@ -110,7 +126,25 @@ struct Query {
     DevBuf<Job> jobs;
     DevBuf<uint32_t> part;     // merge-path partition (DB index at each boundary)
     std::vector<Job> jobs_host;
+    // query length groups per (query file, shard), built when the index has a
+    // genome table (src/fourier/moments.rs:138-177)
+    bool has_groups = false;
+    DevBuf<uint32_t> g_off;    // [nfiles * nshards + 1]
+    DevBuf<uint32_t> g_lint, g_ci, g_shard;
+    DevBuf<double> g_ss, g_ss2, g_cnt;
+    uint64_t ngroups = 0;
 };
+
+// Host helpers shared with the stats code.
+size_t compact_index(size_t l);
+size_t compact_ls(uint32_t n_bins, std::vector<uint32_t>* ls);
+void query_groups(Query& q, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
+                  const uint32_t* qfile);
+void moments_upload(Index& ix, const int64_t* slot_off, const uint32_t* slot_len, uint64_t total,
+                    const double* mean, const double* var);
+// p-values for every cell with count > 0 (device arrays of nfiles(q) x F).
+void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, double* d_p,
+                 double* d_llr, uint8_t* d_flags);
 
 void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                  const uint32_t* fid);

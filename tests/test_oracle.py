@@ -58,3 +58,71 @@ def test_jump_table(orc):
     assert orc.build_jump_table([10, 50, 80], 100).tolist() == [0]
     assert orc.build_jump_table([10, 150, 250], 100).tolist() == [0, 1, 2]
     assert orc.build_jump_table([10, 350], 100).tolist() == [0, 1, 1, 1]
+
+
+def test_stats_known_answers(orc):
+    # src/fourier/stats.rs:218-242
+    assert abs(orc.nb_llr(4000.0, 8000.0, 4000.0)) < 1e-6
+    assert orc.nb_llr(4000.0, 8000.0, 20_000.0) > 0
+    assert orc.nb_llr(2000.0, 1000.0, 1.0) is None
+    assert abs(orc.erfc_as(0.0) - 1.0) < 1e-6
+    assert abs(orc.erfc_as(1.0) - 0.1572992) < 1e-5
+    assert orc.erfc_as(5.0) < 1e-10
+    # A&S 7.1.26 against the true erfc (error < 1.5e-7)
+    import math
+    for x in (0.1, 0.5, 1.0, 2.0, 3.0):
+        assert abs(orc.erfc_as(x) - math.erfc(x)) < 1.5e-7
+
+
+def test_stats_fixed_moments(orc):
+    # stats.rs:197-216: one 10-bin interval, moments (20, 100) everywhere
+    g = orc.group_lengths([1000])
+    groups = g + (np.zeros(1, np.int32),)
+    table = [(508_185, np.full(2000, 20.0), np.full(2000, 100.0))]
+    p0, _ = orc.analytic_stats(groups, None, table, 0.0)
+    assert p0 == 1.0
+    p10, llr = orc.analytic_stats(groups, None, table, 10.0)
+    assert 0.5 < p10 < 1.0 and llr is not None
+
+
+def test_compact_table_layout(orc):
+    ls = orc.compact_ls(1_000_000)
+    assert list(ls[:100]) == list(range(1, 101))
+    # every stored L maps to its own slot; intermediate L to the slot below
+    for i, L in enumerate(ls):
+        assert orc.compact_index(int(L)) == i
+    assert orc.compact_index(102) == 100 and orc.compact_index(103) == 101
+
+
+def test_depth_and_moment_closed_forms(orc):
+    # depth.rs:222-230: [20,80) covers 0.6 of a bin
+    assert orc.depth_signal_f32([20], [80], 10)[0] == pytest.approx(0.6)
+    assert orc.depth_exact([20], [80], 10)[0] == 60
+    # moments.rs:474-488 autocorrelation known answers
+    assert orc.autocorr_fft(np.array([1.0, 2, 3, 0, 0]))[0] == pytest.approx(14.0)
+    assert orc.autocorr_fft(np.array([0.0, 1, 1, 1, 2, 2, 2, 3]))[1] == pytest.approx(18.0)
+    # moments.rs:511-562: uniform g = 1 -> mean L, var 0; periodic [1,0,0]
+    G = np.full(50, 100, np.int64)
+    m, v, _, _ = orc.moments_exact(G, np.arange(1, 50))
+    assert np.allclose(m, np.arange(1, 50)) and np.all(v == 0)
+    m, v, _, _ = orc.moments_exact(np.array([100, 0, 0, 100, 0, 0, 100]), [3, 1])
+    assert m[0] == pytest.approx(1.0) and v[0] == 0 and m[1] == pytest.approx(3 / 7)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_exact_moments_against_window_brute_force(orc, seed):
+    rng = np.random.default_rng(seed)
+    nb = 3000
+    s = rng.integers(0, nb * 100, 400).astype(np.uint32)
+    e = (s + rng.integers(1, 20_000, 400)).astype(np.uint32)
+    G = orc.depth_exact(s, e, nb)
+    ls = orc.compact_ls(nb)
+    m1, v1, _, _ = orc.moments_exact(G, ls)
+    m2, v2 = orc.moments_exact_fft(G, ls)
+    assert np.array_equal(m1, m2) and np.array_equal(v1, v2)
+    P = np.concatenate([[0], np.cumsum(G)])
+    for i in (0, 7, 99, len(ls) - 1):
+        L = int(ls[i])
+        w = (P[L:] - P[:-L]).astype(np.float64) / 100.0
+        assert m1[i] == pytest.approx(w.mean(), rel=1e-12)
+        assert v1[i] == pytest.approx(w.var(), rel=1e-9, abs=1e-9)
