@@ -111,6 +111,23 @@ int chk_moments_upload(chk_index* idx, const int64_t* slot_off, const uint32_t* 
                        uint64_t total, const double* mean, const double* var);
 
 /*
+ * Build the compact moment tables of every (DB file, shard) with intervals on
+ * a genome shard on the device (replaces build_chrom_moments and the moment
+ * store of `chuckle add --stats`, src/tasks/build.rs:160-194,
+ * src/fourier/moments.rs:129-254): exact integer coverage per 100-bp bin,
+ * FFT autocorrelation of the centred signal (LDS-staged radix-2, FP64),
+ * double-double prefix sums, mean/var per stored block size.
+ * exact_rows (optional) receives the number of signals whose autocorrelation
+ * was rounded to exact integers, rows the number of signals.
+ */
+int chk_moments_build(chk_index* idx, int32_t* rows, int32_t* exact_rows);
+
+/* Copy the installed tables to the host: *total receives the entry count;
+ * arrays may be NULL (slot arrays hold nfiles*nshards entries). */
+int chk_moments_download(chk_index* idx, uint64_t* total, int64_t* slot_off, uint32_t* slot_len,
+                         double* mean, double* var);
+
+/*
  * Analytic p-values (replaces compute_analytic_pvalues +
  * compute_analytic_stats, src/tasks/query/pvalues.rs:22-77,
  * src/fourier/stats.rs:34-126) for every cell with count > 0 of a sweep

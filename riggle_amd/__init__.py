@@ -66,6 +66,8 @@ def _declare(L):
     L.chk_index_set_genome.argtypes = [vp, vp]
     L.chk_moments_upload.argtypes = [vp, vp, vp, u64, vp, vp]
     L.chk_pvalues_async.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.chk_moments_build.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.chk_moments_download.argtypes = [vp, C.POINTER(u64), vp, vp, vp, vp]
     L.chk_query_stats.argtypes = [vp, vp, vp, vp, vp, vp, vp]
 
 
@@ -154,6 +156,27 @@ class Index:
         _check(lib().chk_moments_upload(self._h, so.ctypes.data_as(C.c_void_p),
                                         sl.ctypes.data_as(C.c_void_p), len(m),
                                         m.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p)))
+
+    def build_moments(self):
+        """Compute the compact moment tables on the device; returns
+        (signals, signals with exact integer autocorrelation)."""
+        rows, ex = C.c_int32(), C.c_int32()
+        _check(lib().chk_moments_build(self._h, C.byref(rows), C.byref(ex)))
+        return rows.value, ex.value
+
+    def download_moments(self):
+        """(slot_off [F*S], slot_len [F*S], mean, var) of the installed tables."""
+        tot = C.c_uint64()
+        _check(lib().chk_moments_download(self._h, C.byref(tot), None, None, None, None))
+        ns = self.nfiles * self.nshards
+        so = np.zeros(ns, np.int64)
+        sl = np.zeros(ns, np.uint32)
+        m = np.zeros(max(tot.value, 1), np.float64)
+        v = np.zeros(max(tot.value, 1), np.float64)
+        _check(lib().chk_moments_download(self._h, C.byref(tot), so.ctypes.data_as(C.c_void_p),
+                                          sl.ctypes.data_as(C.c_void_p), m.ctypes.data_as(C.c_void_p),
+                                          v.ctypes.data_as(C.c_void_p)))
+        return so, sl, m[:tot.value], v[:tot.value]
 
     def stats(self, query):
         """Sweep + p-values: dict of [query files x DB files] arrays."""

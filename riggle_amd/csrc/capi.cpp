@@ -199,6 +199,27 @@ int chk_moments_upload(chk_index* idx, const int64_t* slot_off, const uint32_t* 
     });
 }
 
+int chk_moments_build(chk_index* idx, int32_t* rows, int32_t* exact_rows) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");
+        set_device(idx->ix.device);
+        chk::moments_build(idx->ix);
+        if (rows) *rows = idx->ix.mom.stat_rows;
+        if (exact_rows) *exact_rows = idx->ix.mom.stat_exact_rows;
+    });
+}
+
+int chk_moments_download(chk_index* idx, uint64_t* total, int64_t* slot_off, uint32_t* slot_len,
+                         double* mean, double* var) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");
+        set_device(idx->ix.device);
+        CHK_REQUIRE(idx->ix.mom.present, CHK_ERR_NOT_FOUND, "the index has no moment tables");
+        if (total) *total = idx->ix.mom.total;
+        chk::moments_download(idx->ix, slot_off, slot_len, mean, var);
+    });
+}
+
 int chk_pvalues_async(chk_index* idx, chk_query* q, const uint64_t* d_counts, const uint64_t* d_bp,
                       double* d_p, double* d_llr, uint8_t* d_flags) {
     return guarded([&] {

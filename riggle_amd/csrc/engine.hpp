@@ -69,6 +69,7 @@ struct Moments {
     DevBuf<uint8_t> d_has_entry;       // [nfiles]: file has a stored entry
     DevBuf<double> mean, var;
     uint64_t total = 0;
+    int stat_rows = 0, stat_exact_rows = 0;  // signals built / whose FFT rounded exactly
 };
 
 // Per-shard sorted streams of the DB (structure of arrays in HBM).
@@ -81,6 +82,7 @@ struct Index {
     uint32_t nshards = 0, nfiles = 0;
     uint64_t n = 0;
     std::vector<uint64_t> shard_off;  // nshards+1, offsets into the streams
+    std::vector<uint32_t> fs_count;   // [nfiles * nshards] intervals per (file, shard)
     DevBuf<int32_t> s_coord, e_coord;  // starts / ends sorted per shard
     DevBuf<uint32_t> s_fid, e_fid;     // file id in the same order (u32 form)
     DevBuf<uint16_t> s_fid16, e_fid16; // ... u16 form, used when nfiles <= 65536
@@ -142,6 +144,9 @@ void query_groups(Query& q, const uint32_t* shard, const uint32_t* start, const 
                   const uint32_t* qfile);
 void moments_upload(Index& ix, const int64_t* slot_off, const uint32_t* slot_len, uint64_t total,
                     const double* mean, const double* var);
+// Compact moment tables of every (file, shard) from the index's intervals.
+void moments_build(Index& ix);
+void moments_download(Index& ix, int64_t* slot_off, uint32_t* slot_len, double* mean, double* var);
 // p-values for every cell with count > 0 (device arrays of nfiles(q) x F).
 void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, double* d_p,
                  double* d_llr, uint8_t* d_flags);

@@ -85,6 +85,7 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
                  const uint32_t* fid) {
     const uint64_t n = ix.n;
     ix.shard_off.assign(ix.nshards + 1, 0);
+    ix.fs_count.assign((size_t)ix.nfiles * ix.nshards, 0);
     for (uint64_t i = 0; i < n; i++) {
         CHK_REQUIRE(shard[i] < ix.nshards, CHK_ERR_INVALID, "interval shard id out of range");
         CHK_REQUIRE(fid[i] < ix.nfiles, CHK_ERR_INVALID, "interval file id out of range");
@@ -92,6 +93,7 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
         CHK_REQUIRE(end[i] <= 0x7FFFFFFFu, CHK_ERR_INVALID,
                     "interval coordinate exceeds 2^31-1 (device coordinates are int32)");
         ix.shard_off[shard[i] + 1]++;
+        ix.fs_count[(size_t)fid[i] * ix.nshards + shard[i]]++;
     }
     for (uint32_t s = 0; s < ix.nshards; s++) ix.shard_off[s + 1] += ix.shard_off[s];
     ix.s_coord.reserve(n);

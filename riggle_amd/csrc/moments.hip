@@ -1,0 +1,675 @@
+// moments.hip -- sliding-window coverage moments of the DB files, computed on
+// the GPU (replaces build_chrom_moments, src/fourier/moments.rs:129-254, and
+// the depth signal of src/fourier/depth.rs:109-136, 206-215).
+//
+// Per (DB file, shard) with n = n_bins bins:
+//   G[j] = bp coverage of bin j (integer; the reference's g = G/100), built
+//          from the fractional impulse train of every interval, spikes
+//          clipped at n*100;
+//   H = G - c, c = round(mean G);  A[m] = sum_k H[k] H[k+m]  (FFT);
+//   S1(L) = PP[n+1] - PP[L] - PP[n-L+1]                (P, PP prefix sums of G)
+//   S2_H(L) = L A[0] + 2 (L C1[L] - C2[L]) - E1[L] - (Q[n] - Q[n-L+1])
+//     C1[L] = sum_{1<=m<L} A[m], C2[L] = sum_{1<=m<L} m A[m],
+//     E1[L] = sum_{1<=k<L} PH[k]^2, Q[i] = sum_{j<i} (PH[n] - PH[j])^2,
+//     PH = prefix sums of H;
+//   mean(L) = S1/(100 n_w),  var(L) = (S2_H n_w - S1_H^2) / (1e4 n_w^2),
+//   S1_H = S1 - c L n_w, n_w = n - L + 1 (the variance is shift-invariant, so
+//   centring removes its cancellation).
+// The autocorrelation is one complex FFT of the zero-padded H of length
+// N = next_pow2(2n - 1), |X|^2, and the inverse FFT, written as LDS-staged
+// radix-2 passes: one block per signal when N <= FFT_SMALL, otherwise the
+// four-step split N = N1 * N2 into column FFTs (length N2, twiddled), row
+// FFTs with |X|^2 and the inverse row FFTs fused, and inverse column FFTs.
+// A is FP64; when every value is within 0.25 of an integer and A[0] equals
+// sum H^2 it is rounded to the exact integers.  Prefix sums are double-double
+// (two-sum compensated), so S2 keeps ~32 significant digits.  Tables are
+// written for the reference's stored block sizes (src/fourier/moments.rs:225-247).
+#include <algorithm>
+#include <cmath>
+
+#include "engine.hpp"
+
+namespace chk {
+namespace {
+
+constexpr int NT = 256;
+constexpr uint32_t FFT_MAXM = 8192;  // largest LDS-resident FFT (complex FP64)
+constexpr uint32_t TW = FFT_MAXM;    // twiddle table period
+
+// ------------------------------------------------------------ double-double
+struct dd {
+    double hi, lo;
+};
+__device__ inline dd two_sum(double a, double b) {
+    double s = a + b, bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+__device__ inline dd dd_add(dd a, dd b) {
+    dd s = two_sum(a.hi, b.hi);
+    double lo = s.lo + a.lo + b.lo;
+    double hi = s.hi + lo;
+    return {hi, lo - (hi - s.hi)};
+}
+__device__ inline dd dd_neg(dd a) { return {-a.hi, -a.lo}; }
+__device__ inline dd dd_from_prod(double a, double b) {  // exact product
+    double p = a * b;
+    return {p, fma(a, b, -p)};
+}
+__device__ inline dd dd_mul_d(dd a, double b) {
+    dd p = dd_from_prod(a.hi, b);
+    double lo = p.lo + a.lo * b;
+    double hi = p.hi + lo;
+    return {hi, lo - (hi - p.hi)};
+}
+__device__ inline dd dd_mul(dd a, dd b) {
+    dd p = dd_from_prod(a.hi, b.hi);
+    double lo = p.lo + a.hi * b.lo + a.lo * b.hi;
+    double hi = p.hi + lo;
+    return {hi, lo - (hi - p.hi)};
+}
+__device__ inline dd dd_from_i64(int64_t v) {  // exact
+    double hi = (double)v;
+    return {hi, (double)(v - (int64_t)hi)};
+}
+__device__ inline dd dd_from_i128(__int128 v) {
+    double hi = (double)v;
+    __int128 r = v - (__int128)hi;
+    return {hi, (double)r};
+}
+__device__ inline double dd_div_to_d(dd a, double b) { return (a.hi + a.lo) / b; }
+
+__device__ inline uint32_t bitrev(uint32_t i, uint32_t logm) {
+    return logm ? __brev(i) >> (32 - logm) : 0u;
+}
+
+// In-place radix-2 DIT FFTs of nfft stacked length-M signals in LDS, input in
+// bit-reversed order, output in natural order.  tw[t] = exp(-2 pi i t / TW).
+template <bool INV>
+__device__ void lds_fft(double2* x, uint32_t M, uint32_t logM, uint32_t nfft, const double2* tw) {
+    const uint32_t nb = nfft * (M >> 1);
+    for (uint32_t s = 1; s <= logM; s++) {
+        const uint32_t half = 1u << (s - 1);
+        const uint32_t stride = TW >> s;  // TW / (2*half)
+        for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
+            const uint32_t f = j >> (logM - 1);
+            const uint32_t jj = j & ((M >> 1) - 1);
+            const uint32_t k = jj & (half - 1);
+            const uint32_t i0 = f * M + ((jj >> (s - 1)) << s) + k;
+            const uint32_t i1 = i0 + half;
+            double2 w = tw[k * stride];
+            if (INV) w.y = -w.y;
+            const double2 a = x[i0], b = x[i1];
+            const double2 bw = {b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x};
+            x[i0] = {a.x + bw.x, a.y + bw.y};
+            x[i1] = {a.x - bw.x, a.y - bw.y};
+        }
+        __syncthreads();
+    }
+}
+
+// exp(-+2 pi i e / N) for e < N (four-step twiddles)
+__device__ inline double2 twiddle(uint64_t e, uint64_t N, bool inv) {
+    double s, c;
+    sincospi(-2.0 * (double)e / (double)N, &s, &c);
+    return {c, inv ? -s : s};
+}
+
+struct Rows {
+    const int32_t* G;      // [B][ldg] coverage per bin
+    uint32_t ldg;
+    const int64_t* cen;    // [B] centring constant c
+    double2* X;            // [B][N] FFT workspace
+    double* A;             // [B][n] autocorrelation output
+    uint32_t n, N, logN;
+    const double2* tw;
+};
+
+// Whole autocorrelation of one signal in LDS (N <= FFT_MAXM).
+__global__ __launch_bounds__(NT) void autocorr_small(Rows r) {
+    extern __shared__ __align__(16) double2 xs[];
+    const uint32_t b = blockIdx.x;
+    const int32_t* G = r.G + (size_t)b * r.ldg;
+    const double c = (double)r.cen[b];
+    for (uint32_t i = threadIdx.x; i < r.N; i += NT)
+        xs[bitrev(i, r.logN)] = {i < r.n ? (double)G[i] - c : 0.0, 0.0};
+    __syncthreads();
+    lds_fft<false>(xs, r.N, r.logN, 1, r.tw);
+    for (uint32_t i = threadIdx.x; i < r.N; i += NT) {
+        const double2 v = xs[i];
+        xs[i] = {v.x * v.x + v.y * v.y, 0.0};
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < r.N; i += NT) {  // back to bit-reversed order
+        const uint32_t j = bitrev(i, r.logN);
+        if (i < j) {
+            const double2 t = xs[i];
+            xs[i] = xs[j];
+            xs[j] = t;
+        }
+    }
+    __syncthreads();
+    lds_fft<true>(xs, r.N, r.logN, 1, r.tw);
+    double* A = r.A + (size_t)b * r.n;
+    for (uint32_t i = threadIdx.x; i < r.n; i += NT) A[i] = xs[i].x / (double)r.N;
+}
+
+// Four-step pass 1: CB adjacent columns n1 of one signal, length-N2 FFTs over
+// n2 (element n1 + N1*n2), twiddle W_N^(n1 k2), stored at n1 + N1*k2.
+__global__ __launch_bounds__(NT) void fft_cols_fwd(Rows r, uint32_t N1, uint32_t N2,
+                                                   uint32_t logN2, uint32_t CB) {
+    extern __shared__ __align__(16) double2 xs[];
+    const uint32_t ncb = N1 / CB;
+    const uint32_t b = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
+    const int32_t* G = r.G + (size_t)b * r.ldg;
+    const double c = (double)r.cen[b];
+    for (uint32_t t = threadIdx.x; t < CB * N2; t += NT) {
+        const uint32_t cc = t % CB, n2 = t / CB;
+        const uint64_t idx = (uint64_t)(c0 + cc) + (uint64_t)N1 * n2;
+        xs[cc * N2 + bitrev(n2, logN2)] = {idx < r.n ? (double)G[idx] - c : 0.0, 0.0};
+    }
+    __syncthreads();
+    lds_fft<false>(xs, N2, logN2, CB, r.tw);
+    double2* X = r.X + (size_t)b * r.N;
+    for (uint32_t t = threadIdx.x; t < CB * N2; t += NT) {
+        const uint32_t cc = t % CB, k2 = t / CB;
+        const uint32_t n1 = c0 + cc;
+        const double2 v = xs[cc * N2 + k2];
+        const double2 w = twiddle(((uint64_t)n1 * k2) % r.N, r.N, false);
+        X[n1 + (size_t)N1 * k2] = {v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x};
+    }
+}
+
+// Four-step pass 2: one row k2 (contiguous N1): FFT, |X|^2, inverse FFT,
+// twiddle W_N^(-n1 k2), in place.
+__global__ __launch_bounds__(NT) void fft_rows(Rows r, uint32_t N1, uint32_t logN1, uint32_t N2) {
+    extern __shared__ __align__(16) double2 xs[];
+    const uint32_t b = blockIdx.x / N2, k2 = blockIdx.x % N2;
+    double2* X = r.X + (size_t)b * r.N + (size_t)k2 * N1;
+    for (uint32_t i = threadIdx.x; i < N1; i += NT) xs[bitrev(i, logN1)] = X[i];
+    __syncthreads();
+    lds_fft<false>(xs, N1, logN1, 1, r.tw);
+    for (uint32_t i = threadIdx.x; i < N1; i += NT) {
+        const double2 v = xs[i];
+        xs[i] = {v.x * v.x + v.y * v.y, 0.0};
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < N1; i += NT) {
+        const uint32_t j = bitrev(i, logN1);
+        if (i < j) {
+            const double2 t = xs[i];
+            xs[i] = xs[j];
+            xs[j] = t;
+        }
+    }
+    __syncthreads();
+    lds_fft<true>(xs, N1, logN1, 1, r.tw);
+    for (uint32_t n1 = threadIdx.x; n1 < N1; n1 += NT) {
+        const double2 v = xs[n1];
+        const double2 w = twiddle(((uint64_t)n1 * k2) % r.N, r.N, true);
+        X[n1] = {v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x};
+    }
+}
+
+// Four-step pass 3: CB columns, inverse length-N2 FFTs over k2, natural n2;
+// A[n1 + N1*n2] = Re / N for indices below n.
+__global__ __launch_bounds__(NT) void fft_cols_inv(Rows r, uint32_t N1, uint32_t N2,
+                                                   uint32_t logN2, uint32_t CB) {
+    extern __shared__ __align__(16) double2 xs[];
+    const uint32_t ncb = N1 / CB;
+    const uint32_t b = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
+    const double2* X = r.X + (size_t)b * r.N;
+    for (uint32_t t = threadIdx.x; t < CB * N2; t += NT) {
+        const uint32_t cc = t % CB, k2 = t / CB;
+        xs[cc * N2 + bitrev(k2, logN2)] = X[(c0 + cc) + (size_t)N1 * k2];
+    }
+    __syncthreads();
+    lds_fft<true>(xs, N2, logN2, CB, r.tw);
+    double* A = r.A + (size_t)b * r.n;
+    for (uint32_t t = threadIdx.x; t < CB * N2; t += NT) {
+        const uint32_t cc = t % CB, n2 = t / CB;
+        const uint64_t idx = (uint64_t)(c0 + cc) + (uint64_t)N1 * n2;
+        if (idx < r.n) A[idx] = xs[cc * N2 + n2].x / (double)r.N;
+    }
+}
+
+// ------------------------------------------------------------ binning
+
+template <typename FidT>
+__global__ void impulses(const int32_t* coord, const FidT* fid, uint64_t cnt, int32_t sign,
+                         const int32_t* slot_of_fid, uint32_t n, uint32_t ldg, int32_t* D) {
+    const uint32_t max_bp = n * 100u;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const int32_t s = slot_of_fid[(uint32_t)fid[i]];
+        if (s < 0) continue;
+        uint32_t x = (uint32_t)coord[i];
+        x = x < max_bp ? x : max_bp;
+        const uint32_t bin = x / 100u, rem = x % 100u;
+        int32_t* row = D + (size_t)s * ldg;
+        if (bin < n) atomicAdd(&row[bin], sign * (int32_t)(100u - rem));
+        if (rem > 0 && bin + 1 < n) atomicAdd(&row[bin + 1], sign * (int32_t)rem);
+    }
+}
+
+template <typename T>
+__device__ T block_incl_scan(T v, T* lds, T& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T y = __shfl_up(v, d, 64);
+        if (lane >= d) v += y;
+    }
+    if (lane == 63) lds[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T run = 0;
+        for (int i = 0; i < NT / 64; i++) {
+            T t = lds[i];
+            lds[i] = run;
+            run += t;
+        }
+        lds[NT / 64] = run;
+    }
+    __syncthreads();
+    v += lds[w];
+    total = lds[NT / 64];
+    __syncthreads();
+    return v;
+}
+
+// One block per row: G = inclusive prefix of the impulse row (in place) and
+// the centring constant c = round(sum G / n).
+__global__ __launch_bounds__(NT) void coverage_rows(int32_t* D, uint32_t ldg, uint32_t n,
+                                                    int64_t* cen, int64_t* sumg) {
+    __shared__ int64_t lds[NT / 64 + 1];
+    int32_t* row = D + (size_t)blockIdx.x * ldg;
+    int64_t carry = 0, sg = 0;
+    for (uint32_t base = 0; base < n; base += NT) {
+        const uint32_t i = base + threadIdx.x;
+        int64_t v = i < n ? row[i] : 0;
+        int64_t tot;
+        int64_t incl = block_incl_scan<int64_t>(v, lds, tot) + carry;
+        if (i < n) {
+            row[i] = (int32_t)incl;
+            sg += incl;
+        }
+        carry += tot;
+    }
+    int64_t tot;
+    (void)block_incl_scan<int64_t>(sg, lds, tot);
+    if (threadIdx.x == 0) {
+        sumg[blockIdx.x] = tot;
+        cen[blockIdx.x] = (int64_t)llround((double)tot / (double)n);
+    }
+}
+
+// ------------------------------------------------------------ evaluation
+
+struct EvalArgs {
+    const int32_t* G;
+    uint32_t ldg, n, nl;
+    const double* A;
+    const int64_t* cen;
+    const int64_t* sumg;
+    const int32_t* pmapL;    // [n+2] position -> L index sampled at i = L, or -1
+    const int32_t* pmapM;    // [n+2] position -> L index sampled at i = n-L+1, or -1
+    const uint32_t* ls;      // [nl] stored block sizes
+    const int64_t* out_off;  // [B] first table entry of the row's slot
+    double* mean;
+    double* var;
+    // per row scratch of samples: [B][nl] each
+    int64_t* sPP_L;
+    int64_t* sPP_M;
+    dd* sC1;
+    dd* sC2;
+    dd* sE1;
+    dd* sQ;
+    uint32_t* exact;         // [B] 1 when A was rounded to integers
+};
+
+__device__ inline dd shfl_up_dd(dd v, int d) {
+    return {__shfl_up(v.hi, d, 64), __shfl_up(v.lo, d, 64)};
+}
+
+// Exclusive block scan of a tuple of dd sums and one int64.
+struct Terms {
+    int64_t pp;
+    dd c1, c2, e1, q;
+};
+__device__ inline Terms terms_add(const Terms& a, const Terms& b) {
+    return {a.pp + b.pp, dd_add(a.c1, b.c1), dd_add(a.c2, b.c2), dd_add(a.e1, b.e1),
+            dd_add(a.q, b.q)};
+}
+
+__device__ Terms block_excl_scan_terms(Terms v, Terms* lds, Terms& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    Terms x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        Terms y = {__shfl_up(x.pp, d, 64), shfl_up_dd(x.c1, d), shfl_up_dd(x.c2, d),
+                   shfl_up_dd(x.e1, d), shfl_up_dd(x.q, d)};
+        if (lane >= d) x = terms_add(x, y);
+    }
+    if (lane == 63) lds[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Terms run = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        for (int i = 0; i < NT / 64; i++) {
+            Terms t = lds[i];
+            lds[i] = run;
+            run = terms_add(run, t);
+        }
+        lds[NT / 64] = run;
+    }
+    __syncthreads();
+    // exclusive = inclusive(x) - v  ->  compute as wave-exclusive + wave base
+    Terms wex = {__shfl_up(x.pp, 1, 64), shfl_up_dd(x.c1, 1), shfl_up_dd(x.c2, 1),
+                 shfl_up_dd(x.e1, 1), shfl_up_dd(x.q, 1)};
+    if (lane == 0) wex = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    Terms res = terms_add(wex, lds[w]);
+    total = lds[NT / 64];
+    __syncthreads();
+    return res;
+}
+
+// One block per row: scan of P (prefix of G), the dd prefix sums C1, C2, E1,
+// Q and PP, sampled at the stored block sizes, then the table entries.
+__global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
+    __shared__ Terms lds[NT / 64 + 1];
+    __shared__ int64_t ldsi[NT / 64 + 1];
+    __shared__ uint32_t sh_exact;
+    const uint32_t b = blockIdx.x, n = a.n;
+    const int32_t* G = a.G + (size_t)b * a.ldg;
+    const double* A = a.A + (size_t)b * n;
+    const int64_t c = a.cen[b];
+    const int64_t PHn = a.sumg[b] - c * (int64_t)n;
+    // exactness of the FFT: round when every value is within 0.25 of an integer
+    // and A[0] equals sum H^2
+    if (threadIdx.x == 0) sh_exact = 1;
+    __syncthreads();
+    {
+        double dev = 0.0;
+        for (uint32_t i = threadIdx.x; i < n; i += NT) dev = fmax(dev, fabs(A[i] - rint(A[i])));
+        if (dev > 0.25) atomicAnd(&sh_exact, 0u);
+        dd h2 = {0, 0};
+        for (uint32_t i = threadIdx.x; i < n; i += NT) {
+            const double h = (double)((int64_t)G[i] - c);
+            h2 = dd_add(h2, dd_from_prod(h, h));
+        }
+        // reduce h2 over the block
+        for (int d = 32; d >= 1; d >>= 1) h2 = dd_add(h2, {__shfl_xor(h2.hi, d, 64), __shfl_xor(h2.lo, d, 64)});
+        __shared__ dd part[NT / 64];
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = h2;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            dd s = part[0];
+            for (int i = 1; i < NT / 64; i++) s = dd_add(s, part[i]);
+            if (fabs((s.hi - rint(A[0])) + s.lo) > 0.5) sh_exact = 0;
+        }
+        __syncthreads();
+    }
+    const bool exact = sh_exact != 0;
+    Terms carry = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    int64_t pcarry = 0;
+    // positions 0..n (n+1 values); P[i] = sum_{j<i} G[j]
+    for (uint32_t base = 0; base <= n; base += NT) {
+        const uint32_t i = base + threadIdx.x;
+        const int64_t g = i < n ? (int64_t)G[i] : 0;
+        int64_t gtot;
+        const int64_t gin = block_incl_scan<int64_t>(g, ldsi, gtot);
+        const int64_t P = pcarry + gin - g;  // exclusive
+        pcarry += gtot;
+        Terms t = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        if (i <= n) {
+            t.pp = P;
+            const double ph = (double)(P - c * (int64_t)i);
+            const double qm = (double)(PHn - (P - c * (int64_t)i));
+            if (i < n) t.q = dd_from_prod(qm, qm);
+            if (i >= 1 && i < n) {
+                double av = A[i];
+                if (exact) av = rint(av);
+                t.c1 = {av, 0.0};
+                t.c2 = dd_from_prod(av, (double)i);
+                t.e1 = dd_from_prod(ph, ph);
+            }
+        }
+        Terms tot;
+        Terms ex = block_excl_scan_terms(t, lds, tot);
+        ex = terms_add(ex, carry);
+        if (i <= n) {
+            const int32_t jl = a.pmapL[i], jm = a.pmapM[i];
+            const size_t rb = (size_t)b * a.nl;
+            if (jl >= 0) {
+                a.sPP_L[rb + jl] = ex.pp;
+                a.sC1[rb + jl] = ex.c1;
+                a.sC2[rb + jl] = ex.c2;
+                a.sE1[rb + jl] = ex.e1;
+            }
+            if (jm >= 0) {
+                a.sPP_M[rb + jm] = ex.pp;
+                a.sQ[rb + jm] = ex.q;
+            }
+        }
+        carry = terms_add(carry, tot);
+    }
+    // carry now holds the totals over positions 0..n: PP[n+1], Q[n+1] = Q[n]
+    __syncthreads();
+    if (threadIdx.x == 0) a.exact[b] = exact ? 1u : 0u;
+    const double A0 = exact ? rint(A[0]) : A[0];
+    const size_t rb = (size_t)b * a.nl;
+    const int64_t PPend = carry.pp;
+    const dd Qn = carry.q;
+    for (uint32_t j = threadIdx.x; j < a.nl; j += NT) {
+        const uint64_t L = a.ls[j];
+        const uint64_t nw = (uint64_t)n - L + 1;
+        const __int128 S1 = (__int128)PPend - a.sPP_L[rb + j] - a.sPP_M[rb + j];
+        const __int128 S1H = S1 - (__int128)c * (__int128)L * (__int128)nw;
+        // T = L A0 + 2 (L C1 - C2)
+        dd T = dd_add(dd_from_prod(A0, (double)L),
+                      dd_mul_d(dd_add(dd_mul_d(a.sC1[rb + j], (double)L), dd_neg(a.sC2[rb + j])), 2.0));
+        dd E = dd_add(a.sE1[rb + j], dd_add(Qn, dd_neg(a.sQ[rb + j])));
+        dd S2 = dd_add(T, dd_neg(E));
+        dd s1h = dd_from_i128(S1H);
+        dd num = dd_add(dd_mul_d(S2, (double)nw), dd_neg(dd_mul(s1h, s1h)));
+        const double nwd = (double)nw;
+        double v = dd_div_to_d(num, 1e4 * nwd * nwd);
+        const int64_t o = a.out_off[b] + j;
+        a.mean[o] = (double)S1 / (100.0 * nwd);
+        a.var[o] = v > 0.0 ? v : 0.0;
+    }
+}
+
+inline uint32_t ilog2(uint64_t v) {
+    uint32_t l = 0;
+    while ((1ull << l) < v) l++;
+    return l;
+}
+
+}  // namespace
+
+void moments_build(Index& ix) {
+    Moments& M = ix.mom;
+    CHK_REQUIRE(M.genome, CHK_ERR_INVALID, "set the genome table before building moments");
+    const uint32_t S = ix.nshards, F = ix.nfiles;
+    hipStream_t st = ix.stream;
+    // slots: (file, shard) with intervals on a genome shard
+    std::vector<int64_t> slot_off((size_t)F * S, -1);
+    std::vector<uint32_t> slot_len((size_t)F * S, 0);
+    std::vector<std::vector<uint32_t>> lists(S);
+    uint64_t total = 0;
+    for (uint32_t s = 0; s < S; s++) compact_ls(M.nbins[s], &lists[s]);
+    for (uint32_t f = 0; f < F; f++)
+        for (uint32_t s = 0; s < S; s++) {
+            if (M.nbins[s] == 0 || ix.fs_count[(size_t)f * S + s] == 0) continue;
+            slot_off[(size_t)f * S + s] = (int64_t)total;
+            slot_len[(size_t)f * S + s] = (uint32_t)lists[s].size();
+            total += lists[s].size();
+        }
+    M.mean.reserve(total);
+    M.var.reserve(total);
+    // twiddles exp(-2 pi i t / TW), t < TW/2, from the host C library in FP64
+    std::vector<double2> twh(TW / 2);
+    for (uint32_t t = 0; t < TW / 2; t++) {
+        const double ang = -2.0 * M_PI * (double)t / (double)TW;
+        twh[t] = make_double2(std::cos(ang), std::sin(ang));
+    }
+    DevBuf<double2> tw;
+    tw.reserve(TW / 2);
+    CHK_HIP(hipMemcpyAsync(tw.p, twh.data(), (TW / 2) * sizeof(double2), hipMemcpyHostToDevice, st));
+    DevBuf<int32_t> slot_of_fid, pmapL, pmapM, D;
+    DevBuf<int64_t> cen, sumg, out_off, sPP_L, sPP_M;
+    DevBuf<dd> sC1, sC2, sE1, sQ;
+    DevBuf<uint32_t> d_ls, d_exact;
+    DevBuf<double2> X;
+    DevBuf<double> A;
+    slot_of_fid.reserve(F);
+    std::vector<int32_t> sof(F, -1);
+    int nexact = 0, nrows = 0;
+    // largest LDS-resident FFT: 8192 complex FP64 (128 KiB) where the device
+    // allows that much LDS per block, else 4096
+    int max_lds = 0;
+    CHK_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ix.device));
+    const uint32_t maxm = max_lds >= (int)(FFT_MAXM * sizeof(double2)) ? FFT_MAXM : FFT_MAXM / 2;
+    const int lds_need = (int)(maxm * sizeof(double2));
+    if (lds_need > 64 * 1024) {
+        CHK_HIP(hipFuncSetAttribute((const void*)autocorr_small, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
+        CHK_HIP(hipFuncSetAttribute((const void*)fft_cols_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
+        CHK_HIP(hipFuncSetAttribute((const void*)fft_rows, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
+        CHK_HIP(hipFuncSetAttribute((const void*)fft_cols_inv, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
+    }
+    for (uint32_t s = 0; s < S; s++) {
+        const uint32_t n = M.nbins[s];
+        if (n < 2) continue;  // empty tables (n <= 1 stores nothing)
+        std::vector<uint32_t> files;
+        for (uint32_t f = 0; f < F; f++)
+            if (ix.fs_count[(size_t)f * S + s]) files.push_back(f);
+        if (files.empty()) continue;
+        const auto& ls = lists[s];
+        const uint32_t nl = (uint32_t)ls.size();
+        const uint64_t N = 1ull << ilog2(2ull * n - 1);
+        CHK_REQUIRE(N <= (uint64_t)maxm * maxm / 8, CHK_ERR_INVALID,
+                    "shard too long for the moment FFT");
+        const uint32_t logN = ilog2(N);
+        // sample maps: position L -> index of L; position n-L+1 -> index of L
+        std::vector<int32_t> pl(n + 2, -1), pm(n + 2, -1);
+        for (uint32_t j = 0; j < nl; j++) {
+            pl[ls[j]] = (int32_t)j;
+            pm[n - ls[j] + 1] = (int32_t)j;
+        }
+        pmapL.reserve(n + 2);
+        pmapM.reserve(n + 2);
+        d_ls.reserve(nl);
+        CHK_HIP(hipMemcpyAsync(pmapL.p, pl.data(), (n + 2) * 4, hipMemcpyHostToDevice, st));
+        CHK_HIP(hipMemcpyAsync(pmapM.p, pm.data(), (n + 2) * 4, hipMemcpyHostToDevice, st));
+        CHK_HIP(hipMemcpyAsync(d_ls.p, ls.data(), nl * 4, hipMemcpyHostToDevice, st));
+        // batch of rows bounded by ~4 GiB of FFT workspace
+        const uint32_t B = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(files.size(),
+                                                        (4ull << 30) / (N * sizeof(double2))));
+        const uint32_t ldg = n + 1;
+        D.reserve((size_t)B * ldg);
+        X.reserve((size_t)B * N);
+        A.reserve((size_t)B * n);
+        cen.reserve(B);
+        sumg.reserve(B);
+        out_off.reserve(B);
+        d_exact.reserve(B);
+        sPP_L.reserve((size_t)B * nl);
+        sPP_M.reserve((size_t)B * nl);
+        sC1.reserve((size_t)B * nl);
+        sC2.reserve((size_t)B * nl);
+        sE1.reserve((size_t)B * nl);
+        sQ.reserve((size_t)B * nl);
+        const uint64_t soff = ix.shard_off[s], scnt = ix.shard_off[s + 1] - ix.shard_off[s];
+        for (size_t f0 = 0; f0 < files.size(); f0 += B) {
+            const uint32_t nb = (uint32_t)std::min<size_t>(B, files.size() - f0);
+            std::vector<int64_t> oo(nb);
+            for (uint32_t k = 0; k < nb; k++) {
+                sof[files[f0 + k]] = (int32_t)k;
+                oo[k] = slot_off[(size_t)files[f0 + k] * S + s];
+            }
+            CHK_HIP(hipMemcpyAsync(slot_of_fid.p, sof.data(), F * 4, hipMemcpyHostToDevice, st));
+            CHK_HIP(hipMemcpyAsync(out_off.p, oo.data(), nb * 8, hipMemcpyHostToDevice, st));
+            CHK_HIP(hipMemsetAsync(D.p, 0, (size_t)nb * ldg * 4, st));
+            const unsigned grid = (unsigned)std::min<uint64_t>((scnt + NT - 1) / NT, (uint64_t)ix.ncu * 16);
+            if (grid) {
+                if (ix.fid16) {
+                    hipLaunchKernelGGL(impulses<uint16_t>, dim3(grid), dim3(NT), 0, st, ix.s_coord.p + soff,
+                                       ix.s_fid16.p + soff, scnt, 1, slot_of_fid.p, n, ldg, D.p);
+                    hipLaunchKernelGGL(impulses<uint16_t>, dim3(grid), dim3(NT), 0, st, ix.e_coord.p + soff,
+                                       ix.e_fid16.p + soff, scnt, -1, slot_of_fid.p, n, ldg, D.p);
+                } else {
+                    hipLaunchKernelGGL(impulses<uint32_t>, dim3(grid), dim3(NT), 0, st, ix.s_coord.p + soff,
+                                       ix.s_fid.p + soff, scnt, 1, slot_of_fid.p, n, ldg, D.p);
+                    hipLaunchKernelGGL(impulses<uint32_t>, dim3(grid), dim3(NT), 0, st, ix.e_coord.p + soff,
+                                       ix.e_fid.p + soff, scnt, -1, slot_of_fid.p, n, ldg, D.p);
+                }
+            }
+            hipLaunchKernelGGL(coverage_rows, dim3(nb), dim3(NT), 0, st, D.p, ldg, n, cen.p, sumg.p);
+            CHK_HIP(hipGetLastError());
+            Rows r{D.p, ldg, cen.p, X.p, A.p, n, (uint32_t)N, logN, tw.p};
+            hipEvent_t t0 = ix.prof.begin(st);
+            if (N <= maxm) {
+                hipLaunchKernelGGL(autocorr_small, dim3(nb), dim3(NT), N * sizeof(double2), st, r);
+            } else {
+                const uint32_t N1 = maxm, N2 = (uint32_t)(N / N1);
+                const uint32_t CB = std::min<uint32_t>(N1, maxm / N2);
+                const size_t lds_col = (size_t)CB * N2 * sizeof(double2);
+                hipLaunchKernelGGL(fft_cols_fwd, dim3(nb * (N1 / CB)), dim3(NT), lds_col, st, r, N1, N2,
+                                   ilog2(N2), CB);
+                hipLaunchKernelGGL(fft_rows, dim3(nb * N2), dim3(NT), N1 * sizeof(double2), st, r, N1,
+                                   ilog2(N1), N2);
+                hipLaunchKernelGGL(fft_cols_inv, dim3(nb * (N1 / CB)), dim3(NT), lds_col, st, r, N1, N2,
+                                   ilog2(N2), CB);
+            }
+            CHK_HIP(hipGetLastError());
+            ix.prof.end("moments_fft", t0, st);
+            EvalArgs ea{D.p, ldg, n, nl, A.p, cen.p, sumg.p, pmapL.p, pmapM.p, d_ls.p, out_off.p,
+                        M.mean.p, M.var.p, sPP_L.p, sPP_M.p, sC1.p, sC2.p, sE1.p, sQ.p, d_exact.p};
+            t0 = ix.prof.begin(st);
+            hipLaunchKernelGGL(eval_rows, dim3(nb), dim3(NT), 0, st, ea);
+            CHK_HIP(hipGetLastError());
+            ix.prof.end("moments_eval", t0, st);
+            std::vector<uint32_t> ex(nb);
+            CHK_HIP(hipMemcpyAsync(ex.data(), d_exact.p, nb * 4, hipMemcpyDeviceToHost, st));
+            CHK_HIP(hipStreamSynchronize(st));
+            for (uint32_t k = 0; k < nb; k++) {
+                nexact += ex[k] ? 1 : 0;
+                sof[files[f0 + k]] = -1;
+            }
+            nrows += nb;
+        }
+    }
+    M.stat_exact_rows = nexact;
+    M.stat_rows = nrows;
+    // install slot tables (entries written above)
+    const size_t ns = (size_t)F * S;
+    M.slot_off = slot_off;
+    M.slot_len = slot_len;
+    std::vector<uint8_t> has(F, 0);
+    for (size_t k = 0; k < ns; k++)
+        if (slot_off[k] >= 0) has[k / S] = 1;
+    M.total = total;
+    M.d_slot_off.reserve(ns);
+    M.d_slot_len.reserve(ns);
+    M.d_has_entry.reserve(F);
+    CHK_HIP(hipMemcpyAsync(M.d_slot_off.p, slot_off.data(), ns * 8, hipMemcpyHostToDevice, st));
+    CHK_HIP(hipMemcpyAsync(M.d_slot_len.p, slot_len.data(), ns * 4, hipMemcpyHostToDevice, st));
+    CHK_HIP(hipMemcpyAsync(M.d_has_entry.p, has.data(), F, hipMemcpyHostToDevice, st));
+    CHK_HIP(hipStreamSynchronize(st));
+    M.present = true;
+}
+
+void moments_download(Index& ix, int64_t* slot_off, uint32_t* slot_len, double* mean, double* var) {
+    Moments& M = ix.mom;
+    CHK_REQUIRE(M.present, CHK_ERR_NOT_FOUND, "the index has no moment tables");
+    const size_t ns = (size_t)ix.nfiles * ix.nshards;
+    if (slot_off) std::copy(M.slot_off.begin(), M.slot_off.end(), slot_off);
+    if (slot_len) std::copy(M.slot_len.begin(), M.slot_len.end(), slot_len);
+    (void)ns;
+    if (mean && M.total)
+        CHK_HIP(hipMemcpy(mean, M.mean.p, M.total * 8, hipMemcpyDeviceToHost));
+    if (var && M.total)
+        CHK_HIP(hipMemcpy(var, M.var.p, M.total * 8, hipMemcpyDeviceToHost));
+}
+
+}  // namespace chk

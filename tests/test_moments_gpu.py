@@ -1,0 +1,87 @@
+"""GPU compact moment tables vs the exact-integer oracle.
+
+The oracle evaluates the reference method (src/fourier/moments.rs:192-254) in
+exact integers; the tolerance is 1e-6 relative on every mean and variance
+(1e-12 absolute floor for zero variances).  Shard lengths cover both FFT
+forms: one LDS-resident FFT (n <= 4096 bins) and the four-step passes."""
+import numpy as np
+import pytest
+
+from tests.cases import flatten, random_files
+from tests.stats_cases import oracle_tables
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-6
+
+
+def build(chk, db, F, S, sizes):
+    ix = chk.Index(*db, S, F)
+    ix.set_genome(sizes)
+    rows, exact = ix.build_moments()
+    return ix, rows, exact
+
+
+def compare(ix, orc, db, F, S, sizes):
+    so, sl, m, v = ix.download_moments()
+    oso, osl, om, ov, tabs = oracle_tables(orc, db, F, S, sizes)
+    assert np.array_equal(sl, osl)
+    assert np.array_equal(so >= 0, oso >= 0)
+    worst_m = worst_v = 0.0
+    for k in np.nonzero(so >= 0)[0]:
+        a, b, L = so[k], oso[k], sl[k]
+        gm, gv, xm, xv = m[a:a + L], v[a:a + L], om[b:b + L], ov[b:b + L]
+        if L == 0:
+            continue
+        worst_m = max(worst_m, float(np.max(np.abs(gm - xm) / np.maximum(np.abs(xm), 1e-12))))
+        worst_v = max(worst_v, float(np.max(np.abs(gv - xv) / np.maximum(np.abs(xv), 1e-12))))
+    return worst_m, worst_v
+
+
+@pytest.mark.parametrize("seed,span", [(0, 300_000), (1, 3_000_000), (2, 20_000_000)])
+def test_tables_match_exact_oracle(chk, orc, seed, span):
+    rng = np.random.default_rng(seed)
+    F, S = 5, 3
+    sizes = np.array([span, span // 2 + 77, 0], np.uint64)
+    db = flatten(random_files(rng, F, S, (1, 3000), span, 20_000))
+    ix, rows, exact = build(chk, db, F, S, sizes)
+    assert rows > 0
+    wm, wv = compare(ix, orc, db, F, S, sizes)
+    assert wm <= REL and wv <= REL, (wm, wv)
+
+
+def test_deep_coverage_and_edges(chk, orc):
+    # depth > 1, intervals past the table end (clipped), sub-bin intervals
+    rng = np.random.default_rng(9)
+    S, F = 2, 2
+    sizes = np.array([1_000_050, 5_000], np.uint64)
+    s = rng.integers(0, 1_050_000, 4000).astype(np.uint32)
+    e = (s + rng.integers(1, 200_000, 4000)).astype(np.uint32)
+    sh = (rng.random(4000) < 0.9).astype(np.uint32) ^ 1
+    db = (sh, s, e, rng.integers(0, F, 4000).astype(np.uint32))
+    ix, rows, exact = build(chk, db, F, S, sizes)
+    wm, wv = compare(ix, orc, db, F, S, sizes)
+    assert wm <= REL and wv <= REL, (wm, wv)
+
+
+def test_gpu_moments_drive_pvalues(chk, orc):
+    from tests.stats_cases import oracle_cell_stats, mlog10
+    rng = np.random.default_rng(4)
+    F, nq, S = 4, 2, 3
+    sizes = np.array([2_000_000, 1_000_000, 600_000], np.uint64)
+    db = flatten(random_files(rng, F, S, (100, 600), 600_000, 20_000))
+    q = flatten(random_files(rng, nq, S, (50, 300), 600_000, 20_000))
+    ix, rows, exact = build(chk, db, F, S, sizes)
+    out = ix.stats(ix.query(q[0], q[1], q[2], q[3], nq))
+    _, _, _, _, tabs = oracle_tables(orc, db, F, S, sizes)
+    oc, obp = orc.sweep_ranks(db, q, F, nq, S)
+    n = 0
+    for r in range(nq):
+        for f in range(F):
+            exp = oracle_cell_stats(orc, q, r, f, S, sizes, tabs, obp[r, f] / 100.0) \
+                if oc[r, f] > 0 else None
+            assert bool(out["flags"][r, f] & 1) == (exp is not None)
+            if exp is not None:
+                n += 1
+                assert abs(mlog10(out["p"][r, f]) - mlog10(exp[0])) < 1e-3
+    assert n > 3
