@@ -65,6 +65,102 @@ def cpu_baseline(db, q, nshards, nfiles, target_s=4.0, max_threads=16):
     return rec, cnt, sel
 
 
+def stats_leg(ix, qq, cfg, F, steps):
+    """Moment tables of the whole DB (FFT on the device), then p-values per
+    nonzero cell of the query; returns timings and throughputs."""
+    ix.set_genome(np.array([x for _, x in cfg["genome"]], np.uint64))
+    ix.profile(True)
+    for name in ("moments_fft", "moments_eval"):
+        ix.profile_read(name)
+    t = time.perf_counter()
+    rows, exact = ix.build_moments()
+    t_mom = time.perf_counter() - t
+    k_fft = ix.profile_read("moments_fft")
+    k_eval = ix.profile_read("moments_eval")
+    so, sl, _, _ = ix.download_moments()
+    nb = np.array([(x + 99) // 100 for _, x in cfg["genome"]], np.int64)
+    S = len(nb)
+    bins = int(sum(nb[k % S] for k in np.nonzero(so >= 0)[0]))
+    log(f"moments: {rows} signals ({exact} exact), {bins} bins in {t_mom:.2f}s")
+    # query groups need the genome: recreate the query handle
+    sh, s, e, _ = cfg["query"]
+    q2 = ix.query(sh, s, e)
+    cells = F
+    dc, db_ = ix.alloc(8 * cells), ix.alloc(8 * cells)
+    dp, dl, dfl = ix.alloc(8 * cells), ix.alloc(8 * cells), ix.alloc(cells)
+    ix.sweep_async(q2, dc, db_)
+    ix.pvalues_async(q2, dc, db_, dp, dl, dfl)
+    ix.sync()
+    ix.profile_read("pvalues")
+    t = time.perf_counter()
+    for _ in range(steps):
+        ix.sweep_async(q2, dc, db_)
+        ix.pvalues_async(q2, dc, db_, dp, dl, dfl)
+    ix.sync()
+    ms_step = (time.perf_counter() - t) * 1e3 / steps
+    n_pv, ms_pv = ix.profile_read("pvalues")
+    ix.profile(False)
+    flags = dfl.to_numpy(np.uint8, cells)
+    emitted = int((flags & 1).sum())
+    pv_ms = ms_pv / max(n_pv, 1)
+    return {"moment_signals": rows, "moment_signals_exact": exact, "moment_bins": bins,
+            "moments_build_s": t_mom, "moments_bins_per_s": bins / t_mom,
+            "moments_fft_ms": k_fft[1], "moments_eval_ms": k_eval[1],
+            "pvalue_cells_emitted": emitted, "pvalue_kernel_ms": pv_ms,
+            "pvalues_per_s_kernel": emitted / (pv_ms * 1e-3),
+            "sweep_plus_pvalues_ms_per_step": ms_step,
+            "pvalues_per_s_end_to_end": emitted / (ms_step * 1e-3)}
+
+
+def batched_leg(ix, cfg, F, nshards, D):
+    """256 query files (log-uniform 100-2M intervals, 30 % hotspot) against the
+    same DB: one batched launch vs one plain launch per file."""
+    from riggle_amd import gen
+    t = time.time()
+    g = cfg["genome"]
+    hot = gen.hotspots(g, 0xC40C1E000 + 2)
+    qc = gen.log_uniform_counts(256, 100, 2e6, 0xC40C1E000 + 4, stream=0xC4C4)
+    sh, s, e = gen.intervals(g, int(qc.sum()), 0xC40C1E000 + 4, 2, 1000.0, 1.2,
+                             hotspot_frac=0.3, hot=hot)
+    qf = np.repeat(np.arange(256, dtype=np.uint32), qc)
+    off = np.concatenate([[0], np.cumsum(qc)])
+    log(f"batched: generated 256 files, {int(qc.sum())} intervals in {time.time() - t:.1f}s")
+    B = 256
+    qb = ix.query(sh, s, e, qf, B)
+    dc, db_ = ix.alloc(8 * B * F), ix.alloc(8 * B * F)
+    ix.sweep_async(qb, dc, db_)
+    ix.sync()
+    reps = 3
+    t = time.perf_counter()
+    for _ in range(reps):
+        ix.sweep_async(qb, dc, db_)
+    ix.sync()
+    ms_batched = (time.perf_counter() - t) * 1e3 / reps
+    cb = dc.to_numpy(np.uint64, (B, F))
+    # plain: one query handle and launch per file
+    singles = [ix.query(sh[off[i]:off[i + 1]], s[off[i]:off[i + 1]], e[off[i]:off[i + 1]])
+               for i in range(B)]
+    d1, d2 = ix.alloc(8 * F), ix.alloc(8 * F)
+    ix.sweep_async(singles[0], d1, d2)
+    ix.sync()
+    t = time.perf_counter()
+    rows_ok = True
+    for i in range(B):
+        ix.sweep_async(singles[i], d1, d2)
+    ix.sync()
+    ms_plain = (time.perf_counter() - t) * 1e3
+    # spot-check rows of the batched matrix against single launches
+    for i in (0, 1, B // 2, B - 1):
+        ix.sweep_async(singles[i], d1, d2)
+        rows_ok &= bool((d1.to_numpy(np.uint64, F) == cb[i]).all())
+    swept = B * D + int(qc.sum())
+    return {"query_files": B, "query_intervals": int(qc.sum()), "intervals_swept": swept,
+            "batched_ms": ms_batched, "plain_ms_256_launches": ms_plain,
+            "batched_swept_per_s": swept / (ms_batched * 1e-3),
+            "plain_swept_per_s": swept / (ms_plain * 1e-3),
+            "batched_over_plain": ms_plain / ms_batched, "rows_match_single_launches": rows_ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +170,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tune", action="append", default=[],
                     help="launch-shape knob key=value (see chk_tune), may repeat")
+    ap.add_argument("--skip-stats", action="store_true", help="skip moments + p-value legs")
+    ap.add_argument("--skip-batched", action="store_true", help="skip the batched 256-file leg")
     ap.add_argument("--calibrate-bytes", type=int, default=0,
                     help="also run the plain copy kernel over this many bytes (counter scaling)")
     args = ap.parse_args()
@@ -157,6 +255,11 @@ def main():
     }
     if calib:
         rec["calibration_copy"] = calib
+
+    if not args.skip_stats:
+        rec["stats"] = stats_leg(ix, qq, cfg, F, args.steps)
+    if not args.skip_batched and args.config == "c2":
+        rec["batched"] = batched_leg(ix, cfg, F, nshards, D)
 
     if not args.no_cpu:
         rec_cpu, cnt_cpu, sel = cpu_baseline((sh, s, e, fid), (qsh, qs, qe), nshards, F)
