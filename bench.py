@@ -65,6 +65,20 @@ def cpu_baseline(db, q, nshards, nfiles, target_s=4.0, max_threads=16):
     return rec, cnt, sel
 
 
+def profiled_traffic(workload):
+    """HBM bytes per sweep launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes over this same bench command (scaled by the plain copy kernel's
+    reading), as committed in profiles/; None when absent."""
+    path = os.path.join(ROOT, "profiles", f"sweep_{workload}_plain.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def stats_leg(ix, qq, cfg, F, steps):
     """Moment tables of the whole DB (FFT on the device), then p-values per
     nonzero cell of the query; returns timings and throughputs."""
@@ -245,7 +259,8 @@ def main():
                    "db_intervals": D, "query_intervals": Q, "db_files": F,
                    "genome_bp": int(sum(x for _, x in cfg["genome"])), "chromosomes": nshards},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(args.config),
                      "kernel": "sweep_kernel", "kernel_ms": sweep_ms,
                      "alg_bytes_per_launch": alg_bytes},
         "kernels_ms_per_step": {k: (v[1] / max(v[0], 1)) for k, v in kern.items()},
