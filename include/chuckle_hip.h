@@ -54,10 +54,12 @@ int chk_device_count(int* count);
  *   shard[i] < nshards  -- shard (chromosome) id of interval i
  *   start[i] < end[i] <= 2^31-1  -- half-open coordinates
  *   fid[i]   < nfiles   -- DB file id (D_SID)
- * Device layout: per shard, a start stream (int32 start sorted ascending +
- * u32 file id) and an end stream (int32 end sorted ascending + u32 file id).
- * Replaces the reference's layer files + jump tables: the merge-path sweep
- * needs neither size classes nor look-back.
+ * Device layout: per (shard, group of 1024 files), a start stream (int32
+ * start ascending + u16 group-local file id) and an end stream (int32 end
+ * ascending + u16 id) for the dense merge-path sweep; per (shard, size
+ * layer), the intervals sorted by start (int32 start, int32 end, u32 file id)
+ * with a jump table for the sparse pair path -- the reference's layer files
+ * and jump tables (src/io/layer.rs:34-128, src/io/jump.rs:19-113) in HBM.
  */
 int chk_index_create(int device, uint32_t nshards, uint32_t nfiles, uint64_t n,
                      const uint32_t* shard, const uint32_t* start, const uint32_t* end,
@@ -160,9 +162,17 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
 /*
  * Launch-shape knobs of the sweep, set by the autotuner or by tests
  * (value 0 restores the automatic choice):
- *   "sweep.blocks_per_cu"   resident blocks per CU the persistent grid targets
- *   "sweep.tiles_per_block" contiguous merge-path tiles per block
- *   "sweep.part_window"     partition entries staged in LDS at a time (2..1024)
+ *   "sweep.blocks_per_cu"  resident dense-sweep blocks per CU (persistent grid)
+ *   "sweep.chunk"          merge-path tiles per work-queue item (1..64)
+ *   "sweep.items"          DB elements per thread per tile: 4, 8 or 16
+ *                          (tile = 1024, 2048 or 4096 merged items)
+ *   "sweep.sparse_items"   query intervals per sparse work item
+ *   "sweep.sparse_max"     largest query file (intervals) swept by the sparse
+ *                          pair path; larger files take the dense path
+ *                          (0: |DB| / 1024)
+ *   "sweep.method"         0 choose per file by size, 1 dense only, 2 sparse only
+ * sparse_max and method apply to query handles created afterwards; the
+ * others to every later sweep.
  */
 int chk_tune(chk_index* idx, const char* key, int64_t value);
 

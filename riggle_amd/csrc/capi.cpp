@@ -151,8 +151,8 @@ int chk_sweep_async(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* 
         CHK_REQUIRE(idx && q && d_counts && d_bp, CHK_ERR_INVALID, "NULL argument");
         CHK_REQUIRE(q->q.idx == &idx->ix, CHK_ERR_INVALID, "query belongs to another index");
         set_device(idx->ix.device);
-        chk::query_prepare(q->q);
-        chk::sweep_run(q->q, d_counts, d_bp);
+        chk::query_plan(q->q);
+        chk::sweep_run(q->q, d_counts, d_bp, idx->ix.stream);
     });
 }
 
@@ -165,8 +165,8 @@ int chk_sweep(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp) {
         chk::DevBuf<uint64_t> dc, db;
         dc.reserve(cells);
         db.reserve(cells);
-        chk::query_prepare(q->q);
-        chk::sweep_run(q->q, dc.p, db.p);
+        chk::query_plan(q->q);
+        chk::sweep_run(q->q, dc.p, db.p, idx->ix.stream);
         CHK_HIP(hipMemcpyAsync(counts, dc.p, cells * 8, hipMemcpyDeviceToHost, idx->ix.stream));
         CHK_HIP(hipMemcpyAsync(bp, db.p, cells * 8, hipMemcpyDeviceToHost, idx->ix.stream));
         CHK_HIP(hipStreamSynchronize(idx->ix.stream));
@@ -247,8 +247,8 @@ int chk_query_stats(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp
         dp.reserve(cells);
         dl.reserve(cells);
         df.reserve(cells);
-        chk::query_prepare(q->q);
-        chk::sweep_run(q->q, dc.p, db.p);
+        chk::query_plan(q->q);
+        chk::sweep_run(q->q, dc.p, db.p, idx->ix.stream);
         chk::pvalues_run(q->q, dc.p, db.p, dp.p, dl.p, df.p);
         CHK_HIP(hipMemcpyAsync(counts, dc.p, cells * 8, hipMemcpyDeviceToHost, st));
         CHK_HIP(hipMemcpyAsync(bp, db.p, cells * 8, hipMemcpyDeviceToHost, st));
@@ -336,10 +336,20 @@ int chk_tune(chk_index* idx, const char* key, int64_t value) {
         CHK_REQUIRE(value >= 0 && value <= 0xFFFFFFFFll, CHK_ERR_INVALID, "tuning value out of range");
         std::string k(key);
         chk::Tuning& t = idx->ix.tune;
-        if (k == "sweep.blocks_per_cu") t.blocks_per_cu = (uint32_t)value;
-        else if (k == "sweep.tiles_per_block") t.tiles_per_block = (uint32_t)value;
-        else if (k == "sweep.part_window") t.part_window = (uint32_t)value;
+        const uint32_t v = (uint32_t)value;
+        if (k == "sweep.blocks_per_cu") t.blocks_per_cu = v;
+        else if (k == "sweep.chunk") t.chunk = v;
+        else if (k == "sweep.items") {
+            CHK_REQUIRE(v == 0 || v == 4 || v == 8 || v == 16, CHK_ERR_INVALID,
+                        "sweep.items must be 0, 4, 8 or 16");
+            t.items = v;
+        } else if (k == "sweep.sparse_max") t.sparse_max = (uint64_t)value;
+        else if (k == "sweep.method") {
+            CHK_REQUIRE(v <= 2, CHK_ERR_INVALID, "sweep.method must be 0, 1 or 2");
+            t.method = v;
+        } else if (k == "sweep.sparse_items") t.sparse_items = v;
         else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
+        idx->ix.tune_version++;
     });
 }
 

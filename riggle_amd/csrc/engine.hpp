@@ -49,11 +49,22 @@ struct Profiler {
     }
 };
 
-// Launch-shape knobs of the sweep (0 = choose automatically).
+// DB files per group: the dense sweep keeps one LDS histogram of a group's
+// files per block, so the DB streams are split by (shard, file group) and
+// carry group-local 16-bit file ids.
+constexpr uint32_t FG = 1024;
+// Size classes of the layered start-sorted table (src/io/meta.rs:33-81,
+// LayerConfig::default_genomic: min_size 128, growth factor 4).
+constexpr uint32_t LAYER_MIN = 128, LAYER_GF = 4;
+
+// Launch-shape and method knobs of the sweep (0 = choose automatically).
 struct Tuning {
-    uint32_t blocks_per_cu = 0;    // resident blocks per CU the grid is sized for
-    uint32_t tiles_per_block = 0;  // contiguous tiles per block
-    uint32_t part_window = 0;      // partition entries per LDS window (<= 1024)
+    uint32_t blocks_per_cu = 0;  // resident dense-sweep blocks per CU
+    uint32_t chunk = 0;          // merge-path tiles per work-queue item
+    uint32_t items = 0;          // DB elements per thread per tile: 4, 8 or 16
+    uint64_t sparse_max = 0;     // largest query file (intervals) swept by the sparse path
+    uint32_t method = 0;         // 0 auto by size, 1 dense only, 2 sparse only
+    uint32_t sparse_items = 0;   // query intervals per sparse work item
 };
 
 // Compact sliding-window moment tables of the DB files (FP64, device).
@@ -72,62 +83,103 @@ struct Moments {
     int stat_rows = 0, stat_exact_rows = 0;  // signals built / whose FFT rounded exactly
 };
 
-// Per-shard sorted streams of the DB (structure of arrays in HBM).
+// One (shard, layer) segment of the layered start-sorted table with its jump
+// table: jt[t] = first element whose start >= t << shift (src/io/jump.rs:66-113
+// with a power-of-two bucket sized to the segment).
+struct LayerSeg {
+    uint64_t off;       // first element in l_start / l_end / l_fid
+    uint64_t jt;        // first entry in the jump table
+    uint32_t n;         // elements
+    uint32_t nb;        // jump entries
+    uint32_t shift;     // bucket = start >> shift
+    uint32_t lookback;  // layer_max_size: every length in the layer is below it
+};
+
+// HBM-resident index.  Dense form: per (shard, file group) a start stream
+// (int32 start ascending + u16 group-local file id) and an end stream (int32
+// end ascending + u16 id), 12 bytes per interval.  Sparse form: per (shard,
+// size layer) the intervals sorted by start as int32 start / int32 end / u32
+// file id plus a jump table, 12 bytes per interval + the tables.
 struct Index {
     Tuning tune;
+    uint32_t tune_version = 0;
     Moments mom;
     int device = 0;
     hipStream_t stream = nullptr;
     int ncu = 256;
-    uint32_t nshards = 0, nfiles = 0;
+    uint32_t nshards = 0, nfiles = 0, ngroups = 0;
     uint64_t n = 0;
-    std::vector<uint64_t> shard_off;  // nshards+1, offsets into the streams
     std::vector<uint32_t> fs_count;   // [nfiles * nshards] intervals per (file, shard)
-    DevBuf<int32_t> s_coord, e_coord;  // starts / ends sorted per shard
-    DevBuf<uint32_t> s_fid, e_fid;     // file id in the same order (u32 form)
-    DevBuf<uint16_t> s_fid16, e_fid16; // ... u16 form, used when nfiles <= 65536
-    bool fid16 = false;
+    std::vector<uint64_t> shard_n;    // intervals per shard
+    std::vector<uint64_t> str_off;    // [nshards * ngroups + 1] stream segment offsets
+    DevBuf<int32_t> s_coord, e_coord;
+    DevBuf<uint16_t> s_lfid, e_lfid;
+    uint32_t nlayers = 0;
+    std::vector<LayerSeg> lseg;       // [nshards * nlayers]
+    DevBuf<LayerSeg> d_lseg;
+    DevBuf<int32_t> l_start, l_end;
+    DevBuf<uint32_t> l_fid;
+    DevBuf<uint32_t> jt;
     SortScratch scratch;
     Profiler prof;
     ~Index() {
         if (stream) (void)hipStreamDestroy(stream);
     }
     uint64_t device_bytes() const {
-        return s_coord.bytes() + e_coord.bytes() + s_fid.bytes() + e_fid.bytes() +
-               s_fid16.bytes() + e_fid16.bytes();
+        return s_coord.bytes() + e_coord.bytes() + s_lfid.bytes() + e_lfid.bytes() +
+               l_start.bytes() + l_end.bytes() + l_fid.bytes() + jt.bytes() + d_lseg.bytes();
     }
 };
 
-// One unit of sweep work: a DB stream of one shard merged with one query
-// file's event segment on that shard.
+// One unit of dense sweep work: the start or end stream of one (shard, file
+// group) merged with one query file's event segment on that shard.
 struct Job {
-    uint64_t db_off;   // first element in the stream arrays
-    uint32_t n_db;     // DB elements
-    uint32_t q_off;    // first event of the segment in the sorted event array
-    uint32_t m;        // events in the segment
-    uint32_t qfile;    // output row
-    uint32_t is_end;   // 0 = start stream, 1 = end stream
-    uint32_t tile0;    // first global tile of this job
-    uint32_t part0;    // first partition boundary of this job
-    uint32_t pad;
+    uint64_t db_off;  // first element in the stream arrays
+    uint32_t n_db;    // DB elements
+    uint32_t q_off;   // first event of the segment in the sorted event array
+    uint32_t m;       // events in the segment
+    uint32_t row;     // output row (query file)
+    uint32_t group;   // DB file group
+    uint32_t is_end;  // 0 = start stream, 1 = end stream
+    uint32_t tile0;   // first global tile
+    uint32_t part0;   // first merge-path boundary
+};
+
+// Sparse work item: query intervals [begin, end) (upload order) of one file.
+struct SparseItem {
+    uint32_t row, begin, end, pad;
 };
 
 struct Query {
     Index* idx = nullptr;
     uint32_t nfiles = 0;
-    uint64_t n = 0;
-    DevBuf<uint32_t> shard, start, end, qfile;  // uploaded intervals
-    // prepared form (rebuilt by every sweep)
-    DevBuf<uint64_t> keys;     // 2n event keys, sorted
-    DevBuf<uint32_t> key32;    // their low 32 bits (2*coord + is_start)
-    DevBuf<uint32_t> rs;       // 2n+1 exclusive prefix: # start events
-    DevBuf<int64_t> w;         // 2n+1 exclusive prefix: sum(start) - sum(end)
-    DevBuf<int64_t> w_sums;
-    DevBuf<uint32_t> seg;      // [2][nfiles*nshards] first / one-past-last event
-    std::vector<uint32_t> seg_host;
-    DevBuf<Job> jobs;
-    DevBuf<uint32_t> part;     // merge-path partition (DB index at each boundary)
+    uint64_t n = 0;                   // intervals kept (on shards of the DB)
+    uint64_t n_dense = 0;             // the first n_dense belong to dense files
+    std::vector<uint64_t> file_n;     // intervals per query file
+    std::vector<uint8_t> file_dense;  // method per query file
+    std::vector<uint32_t> dense_rows; // rows of the dense files, in upload order
+    std::vector<uint32_t> seg_n;      // [ndense * nshards] intervals per (dense file, shard)
+    // uploaded intervals, grouped by file (dense files first)
+    DevBuf<uint32_t> shard, start, end, row, segid;
+    uint32_t plan_version = 0xFFFFFFFFu;
+    // dense plan
+    uint32_t tile = 0, chunk = 0, items = 0, ntiles = 0, nbounds = 0, nchunks = 0, grid = 0;
     std::vector<Job> jobs_host;
+    std::vector<uint32_t> chunk_job_host;
+    DevBuf<Job> jobs;
+    DevBuf<uint32_t> chunk_job;
+    DevBuf<uint32_t> part;            // merge-path split (DB index) at each tile boundary
+    DevBuf<uint32_t> queue;           // work-queue counter (zeroed by each sweep; 8 bytes)
+    // sparse plan
+    std::vector<SparseItem> items_host;
+    DevBuf<SparseItem> sitems;
+    // prepared events (rebuilt by every sweep)
+    DevBuf<uint64_t> keys;            // 2 n_dense event keys, sorted
+    DevBuf<uint32_t> key32;           // their low 32 bits (2*coord + is_start)
+    DevBuf<uint32_t> rs;              // exclusive prefix: # start events
+    DevBuf<int64_t> w;                // exclusive prefix: sum(start) - sum(end)
+    DevBuf<int64_t> w_sums;
+    SortScratch scratch;              // per query, so batches on other streams do not share it
     // query length groups per (query file, shard), built when the index has a
     // genome table (src/fourier/moments.rs:138-177)
     bool has_groups = false;
@@ -155,10 +207,14 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
                  const uint32_t* fid);
 void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                   const uint32_t* qfile);
-// Sort events, prefix sums and segment table (device), then the host job list.
-void query_prepare(Query& q);
-// Merge-path partition + sweep into device count / bp matrices (zeroed here).
-void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp);
+// Host work plan of the dense and sparse paths for the current tuning.
+void query_plan(Query& q);
+// One sweep: zero the outputs, sort the dense files' events, prefix sums,
+// merge-path partition, dense sweep, sparse sweep.  Enqueued on `st` with
+// sizes known on the host only, so the sequence can be graph-captured.
+void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st);
+// Fill n u64 with zero from a kernel (graph-safe clearing).
+void zero_u64(uint64_t* p, size_t n, hipStream_t st);
 // Plain copy of `bytes` (reads and writes that many bytes once).
 void calibrate_copy(Index& ix, uint64_t bytes);
 

@@ -1,19 +1,26 @@
-// index.hip -- HBM index build and query-batch preparation.
+// index.hip -- HBM index build and query-batch upload / planning.
 //
-// Index: every DB interval appears once in the start stream of its shard
-// (keyed by start) and once in the end stream (keyed by end); each stream is
-// sorted by (shard, coordinate) with the file id carried along.  This replaces
-// the reference's size-class layers and jump tables (src/io/meta.rs:33-81,
-// src/io/jump.rs:66-113): the sweep ranks DB coordinates against the query's
-// event stream instead of scanning a look-back window per query.
+// Dense form: every DB interval appears once in the start stream of its
+// (shard, file group) and once in the end stream; each stream is sorted by
+// coordinate and carries the group-local file id.  The sweep ranks DB
+// coordinates against a query's sorted event stream, so it needs neither the
+// reference's size classes nor its look-back windows (src/io/meta.rs:33-81,
+// src/io/jump.rs:66-113, src/sweep/query.rs:84-117).
 //
-// Query preparation: each query interval yields a start event (key 2*s+1) and
-// an end event (key 2*e); keys are prefixed by (query file, shard) and sorted.
-// An end at x sorts before a DB coordinate x and a start at x after it, which
-// is exactly the half-open overlap rule d.start < q.end && d.end > q.start of
-// src/sweep/query.rs:148-151.  Exclusive prefix sums over the sorted events
-// give, at any rank p, rS = # starts, rE = p - rS and W = sum(starts) -
-// sum(ends) of the events ranked before p.
+// Sparse form: per (shard, size layer) the intervals sorted by start with
+// their ends and file ids, plus a jump table -- the reference's layer files
+// (src/io/layer.rs:34-128, src/tasks/build.rs:196-247) held in HBM, used to
+// enumerate the overlapping pairs of small query files directly.
+//
+// Query events: each interval of a dense query file yields a start event
+// (key 2*s+1) and an end event (key 2*e) prefixed by (dense file, shard) and
+// sorted.  An end at x sorts before a DB coordinate x (key 2x+1) and a start
+// at x after it, which is exactly the half-open rule d.start < q.end &&
+// d.end > q.start of src/sweep/query.rs:148-151.  Exclusive prefix sums over
+// the sorted events give, at any rank p, rS = # starts, rE = p - rS and
+// W = sum(starts) - sum(ends) of the events ranked before p.
+#include <algorithm>
+
 #include "engine.hpp"
 
 namespace chk {
@@ -21,179 +28,338 @@ namespace {
 
 constexpr int NT = 256;
 
-__global__ void make_index_keys(const uint32_t* shard, const uint32_t* coord, size_t n,
-                                uint64_t* keys) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = ((uint64_t)shard[i] << 31) | (uint64_t)coord[i];
-}
-
-__global__ void split_index_keys(const uint64_t* keys, size_t n, int32_t* coord) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) coord[i] = (int32_t)(keys[i] & 0x7FFFFFFFull);
-}
-
-__global__ void narrow_u16(const uint32_t* src, size_t n, uint16_t* dst) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (uint16_t)src[i];
-}
-
-// Two events per query interval.  Dropped intervals (shard not in the DB)
-// get the all-ones key and sort after every valid event.
-__global__ void make_event_keys(const uint32_t* shard, const uint32_t* start, const uint32_t* end,
-                                const uint32_t* qfile, size_t n, uint32_t nshards, int shard_bits,
-                                uint64_t* keys) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s = shard[i];
-    uint64_t k0 = ~0ull, k1 = ~0ull;
-    if (s < nshards) {
-        uint64_t seg = ((uint64_t)(qfile ? qfile[i] : 0u) << shard_bits) | s;
-        k0 = (seg << 32) | (2ull * start[i] + 1ull);
-        k1 = (seg << 32) | (2ull * end[i]);
-    }
-    keys[2 * i] = k0;
-    keys[2 * i + 1] = k1;
-}
-
-// Segment table: seg[0][g] = first event of segment g, seg[1][g] = one past
-// its last event; g = qfile * nshards + shard.  Also the per-event values to
-// be prefix-summed.
-__global__ void segment_and_values(const uint64_t* keys, size_t m, uint32_t nshards,
-                                   int shard_bits, uint32_t nseg, uint32_t* seg, uint32_t* rs,
-                                   int64_t* w, uint32_t* key32) {
-    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
-    uint64_t key = keys[k];
-    bool valid = key != ~0ull;
-    uint32_t lo = (uint32_t)key;
-    key32[k] = lo;
-    rs[k] = valid ? (lo & 1u) : 0u;
-    int64_t c = (int64_t)(lo >> 1);
-    w[k] = valid ? ((lo & 1u) ? c : -c) : 0;
-    if (!valid) return;
-    uint64_t hi = key >> 32;
-    uint32_t g = (uint32_t)(hi >> shard_bits) * nshards + (uint32_t)(hi & ((1ull << shard_bits) - 1));
-    if (k == 0 || (keys[k - 1] >> 32) != hi) seg[g] = (uint32_t)k;
-    if (k + 1 == m || (keys[k + 1] >> 32) != hi) seg[nseg + g] = (uint32_t)(k + 1);
-}
-
 inline unsigned nblk(size_t n) { return (unsigned)((n + NT - 1) / NT); }
 
+// key = (segment << 31) | coordinate; segment = shard * ngroups + group
+__global__ void make_stream_keys(const uint32_t* shard, const uint32_t* coord, const uint32_t* fid,
+                                 size_t n, uint32_t ngroups, uint64_t* keys, uint32_t* vals) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t f = fid[i];
+    const uint64_t seg = (uint64_t)shard[i] * ngroups + f / FG;
+    keys[i] = (seg << 31) | (uint64_t)coord[i];
+    vals[i] = f % FG;
+}
+
+__global__ void split_stream_keys(const uint64_t* keys, const uint32_t* vals, size_t n,
+                                  int32_t* coord, uint16_t* lfid) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    coord[i] = (int32_t)(keys[i] & 0x7FFFFFFFull);
+    lfid[i] = (uint16_t)vals[i];
+}
+
+// Size class of a length (src/io/meta.rs:33-61 for min 128, factor 4).
+__host__ __device__ inline uint32_t layer_of(uint32_t len) {
+    if (len < LAYER_MIN) return 0;
+    const uint32_t l2 = 31u - (uint32_t)__builtin_clz(len);
+    return (l2 - 7u) / 2u + 1u;
+}
+
+// key = ((shard * nlayers + layer) << 31) | start; payload = interval index
+__global__ void make_layer_keys(const uint32_t* shard, const uint32_t* start, const uint32_t* end,
+                                size_t n, uint32_t nlayers, uint64_t* keys, uint32_t* vals) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t seg = (uint64_t)shard[i] * nlayers + layer_of(end[i] - start[i]);
+    keys[i] = (seg << 31) | (uint64_t)start[i];
+    vals[i] = (uint32_t)i;
+}
+
+__global__ void gather_layer(const uint64_t* keys, const uint32_t* vals, size_t n,
+                             const uint32_t* end, const uint32_t* fid, int32_t* l_start,
+                             int32_t* l_end, uint32_t* l_fid) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = vals[i];
+    l_start[i] = (int32_t)(keys[i] & 0x7FFFFFFFull);
+    l_end[i] = (int32_t)end[k];
+    l_fid[i] = fid[k];
+}
+
+// Jump table of every (shard, layer) segment: element i of a segment writes
+// jt[t] = i for every bucket t in (bucket(i-1), bucket(i)]; the last element
+// also fills the buckets after it with the segment length.
+__global__ void build_jump(const LayerSeg* segs, uint32_t nseg, const int32_t* l_start,
+                           uint64_t n, uint32_t* jt) {
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nseg;  // last segment with off <= g
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (segs[mid].off <= g) lo = mid; else hi = mid;
+        }
+        while (lo + 1 < nseg && segs[lo].n == 0) lo++;  // empty segments share offsets
+        const LayerSeg S = segs[lo];
+        const uint32_t i = (uint32_t)(g - S.off);
+        const uint32_t b = (uint32_t)l_start[g] >> S.shift;
+        const uint32_t bp = i ? ((uint32_t)l_start[g - 1] >> S.shift) + 1u : 0u;
+        for (uint32_t t = bp; t <= b && t < S.nb; t++) jt[S.jt + t] = i;
+        if (i + 1 == S.n)
+            for (uint32_t t = b + 1; t < S.nb; t++) jt[S.jt + t] = S.n;
+    }
+}
+
+// Two events per interval of a dense query file.
+__global__ void make_event_keys(const uint32_t* segid, const uint32_t* start, const uint32_t* end,
+                                size_t n, uint64_t* keys) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t seg = (uint64_t)segid[i] << 32;
+    keys[2 * i] = seg | (2ull * start[i] + 1ull);
+    keys[2 * i + 1] = seg | (2ull * end[i]);
+}
+
+// Per-event values to be prefix-summed and the 32-bit keys; entry m gets the
+// sentinel key above every DB key.
+__global__ void event_values(const uint64_t* keys, size_t m, uint32_t* rs, int64_t* w,
+                             uint32_t* key32) {
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > m) return;
+    if (k == m) {
+        key32[m] = 0xFFFFFFFFu;
+        return;
+    }
+    const uint32_t lo = (uint32_t)keys[k];
+    key32[k] = lo;
+    rs[k] = lo & 1u;
+    const int64_t c = (int64_t)(lo >> 1);
+    w[k] = (lo & 1u) ? c : -c;
+}
+
+__global__ void fill_u64(uint64_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
 }  // namespace
+
+void zero_u64(uint64_t* p, size_t n, hipStream_t st) {
+    if (!n) return;
+    const unsigned g = (unsigned)std::min<size_t>(nblk(n), 4096);
+    hipLaunchKernelGGL(fill_u64, dim3(g), dim3(NT), 0, st, p, n);
+    CHK_HIP(hipGetLastError());
+}
 
 void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                  const uint32_t* fid) {
     const uint64_t n = ix.n;
-    ix.shard_off.assign(ix.nshards + 1, 0);
-    ix.fs_count.assign((size_t)ix.nfiles * ix.nshards, 0);
+    const uint32_t S = ix.nshards;
+    ix.ngroups = std::max<uint32_t>(1, (ix.nfiles + FG - 1) / FG);
+    const uint32_t NG = ix.ngroups;
+    ix.fs_count.assign((size_t)ix.nfiles * S, 0);
+    ix.shard_n.assign(S, 0);
+    ix.str_off.assign((size_t)S * NG + 1, 0);
+    uint32_t maxlayer = 0;
     for (uint64_t i = 0; i < n; i++) {
-        CHK_REQUIRE(shard[i] < ix.nshards, CHK_ERR_INVALID, "interval shard id out of range");
+        CHK_REQUIRE(shard[i] < S, CHK_ERR_INVALID, "interval shard id out of range");
         CHK_REQUIRE(fid[i] < ix.nfiles, CHK_ERR_INVALID, "interval file id out of range");
         CHK_REQUIRE(start[i] < end[i], CHK_ERR_INVALID, "interval start must be less than end");
         CHK_REQUIRE(end[i] <= 0x7FFFFFFFu, CHK_ERR_INVALID,
                     "interval coordinate exceeds 2^31-1 (device coordinates are int32)");
-        ix.shard_off[shard[i] + 1]++;
-        ix.fs_count[(size_t)fid[i] * ix.nshards + shard[i]]++;
+        ix.shard_n[shard[i]]++;
+        ix.str_off[(size_t)shard[i] * NG + fid[i] / FG + 1]++;
+        ix.fs_count[(size_t)fid[i] * S + shard[i]]++;
+        maxlayer = std::max(maxlayer, layer_of(end[i] - start[i]));
     }
-    for (uint32_t s = 0; s < ix.nshards; s++) ix.shard_off[s + 1] += ix.shard_off[s];
+    for (size_t g = 0; g < (size_t)S * NG; g++) ix.str_off[g + 1] += ix.str_off[g];
+    ix.nlayers = n ? maxlayer + 1 : 1;
+    const uint32_t NL = ix.nlayers;
     ix.s_coord.reserve(n);
     ix.e_coord.reserve(n);
-    ix.s_fid.reserve(n);
-    ix.e_fid.reserve(n);
-    if (n == 0) return;
+    ix.s_lfid.reserve(n);
+    ix.e_lfid.reserve(n);
+    ix.l_start.reserve(n);
+    ix.l_end.reserve(n);
+    ix.l_fid.reserve(n);
     hipStream_t st = ix.stream;
-    DevBuf<uint32_t> d_shard, d_coord;
+    // layer segment sizes on the host (for the jump-table geometry)
+    std::vector<uint64_t> lcount((size_t)S * NL, 0);
+    std::vector<uint32_t> lmax((size_t)S * NL, 0);
+    for (uint64_t i = 0; i < n; i++) {
+        const size_t b = (size_t)shard[i] * NL + layer_of(end[i] - start[i]);
+        lcount[b]++;
+        lmax[b] = std::max(lmax[b], start[i]);
+    }
+    ix.lseg.assign((size_t)S * NL, LayerSeg{});
+    uint64_t off = 0, jto = 0;
+    for (size_t b = 0; b < (size_t)S * NL; b++) {
+        LayerSeg& L = ix.lseg[b];
+        const uint32_t layer = (uint32_t)(b % NL);
+        L.off = off;
+        L.n = (uint32_t)lcount[b];
+        CHK_REQUIRE(lcount[b] < (1ull << 32), CHK_ERR_INVALID, "layer segment exceeds 2^32 intervals");
+        uint64_t lb = LAYER_MIN;
+        for (uint32_t k = 0; k < layer; k++) lb = std::min<uint64_t>(lb * LAYER_GF, 0xFFFFFFFFull);
+        L.lookback = (uint32_t)lb;
+        // power-of-two buckets: at most max(256, n/4) of them, none narrower
+        // than half the layer's largest length
+        const uint64_t target = std::max<uint64_t>(256, lcount[b] / 4);
+        uint32_t sh = 0;
+        while ((1ull << sh) < lb / 2) sh++;
+        while (((uint64_t)lmax[b] >> sh) + 1 > target) sh++;
+        L.shift = sh;
+        L.nb = L.n ? (uint32_t)(((uint64_t)lmax[b] >> sh) + 1) : 0;
+        L.jt = jto;
+        off += L.n;
+        jto += L.nb;
+    }
+    ix.jt.reserve(jto);
+    ix.d_lseg.reserve(ix.lseg.size());
+    CHK_HIP(hipMemcpyAsync(ix.d_lseg.p, ix.lseg.data(), ix.lseg.size() * sizeof(LayerSeg),
+                           hipMemcpyHostToDevice, st));
+    if (n == 0) {
+        CHK_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    DevBuf<uint32_t> d_shard, d_start, d_end, d_fid, vals;
     DevBuf<uint64_t> keys;
     d_shard.reserve(n);
-    d_coord.reserve(n);
+    d_start.reserve(n);
+    d_end.reserve(n);
+    d_fid.reserve(n);
+    vals.reserve(n);
     keys.reserve(n);
     CHK_HIP(hipMemcpyAsync(d_shard.p, shard, n * 4, hipMemcpyHostToDevice, st));
-    uint32_t nbits = 31 + bits_for(ix.nshards ? ix.nshards - 1 : 0);
+    CHK_HIP(hipMemcpyAsync(d_start.p, start, n * 4, hipMemcpyHostToDevice, st));
+    CHK_HIP(hipMemcpyAsync(d_end.p, end, n * 4, hipMemcpyHostToDevice, st));
+    CHK_HIP(hipMemcpyAsync(d_fid.p, fid, n * 4, hipMemcpyHostToDevice, st));
+    // dense streams
+    const uint32_t sbits = 31 + bits_for((uint64_t)S * NG - 1);
     for (int pass = 0; pass < 2; pass++) {
-        const uint32_t* coord = pass == 0 ? start : end;
-        int32_t* out_coord = pass == 0 ? ix.s_coord.p : ix.e_coord.p;
-        uint32_t* out_fid = pass == 0 ? ix.s_fid.p : ix.e_fid.p;
-        CHK_HIP(hipMemcpyAsync(d_coord.p, coord, n * 4, hipMemcpyHostToDevice, st));
-        CHK_HIP(hipMemcpyAsync(out_fid, fid, n * 4, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(make_index_keys, dim3(nblk(n)), dim3(NT), 0, st, d_shard.p, d_coord.p, n,
-                           keys.p);
-        radix_sort_u64(keys.p, out_fid, n, nbits, ix.scratch, st);
-        hipLaunchKernelGGL(split_index_keys, dim3(nblk(n)), dim3(NT), 0, st, keys.p, n, out_coord);
+        hipLaunchKernelGGL(make_stream_keys, dim3(nblk(n)), dim3(NT), 0, st, d_shard.p,
+                           pass == 0 ? d_start.p : d_end.p, d_fid.p, (size_t)n, NG, keys.p, vals.p);
+        radix_sort_u64(keys.p, vals.p, n, sbits, ix.scratch, st);
+        hipLaunchKernelGGL(split_stream_keys, dim3(nblk(n)), dim3(NT), 0, st, keys.p, vals.p,
+                           (size_t)n, pass == 0 ? ix.s_coord.p : ix.e_coord.p,
+                           pass == 0 ? ix.s_lfid.p : ix.e_lfid.p);
         CHK_HIP(hipGetLastError());
     }
-    if (ix.nfiles <= 65536) {  // 2-byte file ids: 12 bytes per interval over both streams
-        ix.fid16 = true;
-        ix.s_fid16.reserve(n);
-        ix.e_fid16.reserve(n);
-        hipLaunchKernelGGL(narrow_u16, dim3(nblk(n)), dim3(NT), 0, st, ix.s_fid.p, n, ix.s_fid16.p);
-        hipLaunchKernelGGL(narrow_u16, dim3(nblk(n)), dim3(NT), 0, st, ix.e_fid.p, n, ix.e_fid16.p);
-        CHK_HIP(hipGetLastError());
-        CHK_HIP(hipStreamSynchronize(st));
-        ix.s_fid.release();
-        ix.e_fid.release();
-    }
+    // layered table + jump tables
+    const uint32_t lbits = 31 + bits_for((uint64_t)S * NL - 1);
+    hipLaunchKernelGGL(make_layer_keys, dim3(nblk(n)), dim3(NT), 0, st, d_shard.p, d_start.p,
+                       d_end.p, (size_t)n, NL, keys.p, vals.p);
+    radix_sort_u64(keys.p, vals.p, n, lbits, ix.scratch, st);
+    hipLaunchKernelGGL(gather_layer, dim3(nblk(n)), dim3(NT), 0, st, keys.p, vals.p, (size_t)n,
+                       d_end.p, d_fid.p, ix.l_start.p, ix.l_end.p, ix.l_fid.p);
+    const unsigned gj = (unsigned)std::min<uint64_t>(nblk(n), (uint64_t)ix.ncu * 32);
+    hipLaunchKernelGGL(build_jump, dim3(gj), dim3(NT), 0, st, ix.d_lseg.p, (uint32_t)ix.lseg.size(),
+                       ix.l_start.p, n, ix.jt.p);
+    CHK_HIP(hipGetLastError());
     CHK_HIP(hipStreamSynchronize(st));
     ix.scratch.keys_alt.release();
     ix.scratch.vals_alt.release();
 }
 
+// Upload one batch of query files.  Intervals on shards absent from the DB
+// are dropped (the reference skips such shards, src/tasks/query/mod.rs:193-199);
+// the rest are grouped by file, dense files first, so each sparse file is a
+// contiguous range and the dense files' events are a prefix.
 void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                   const uint32_t* qfile) {
-    const uint64_t n = q.n;
-    for (uint64_t i = 0; i < n; i++) {
+    Index& ix = *q.idx;
+    const uint64_t n_in = q.n;
+    const uint32_t S = ix.nshards, R = q.nfiles;
+    q.file_n.assign(R, 0);
+    for (uint64_t i = 0; i < n_in; i++) {
         CHK_REQUIRE(start[i] < end[i], CHK_ERR_INVALID, "query interval start must be less than end");
         CHK_REQUIRE(end[i] <= 0x7FFFFFFFu, CHK_ERR_INVALID,
                     "query coordinate exceeds 2^31-1 (device coordinates are int32)");
-        if (qfile) CHK_REQUIRE(qfile[i] < q.nfiles, CHK_ERR_INVALID, "query file id out of range");
+        const uint32_t r = qfile ? qfile[i] : 0u;
+        CHK_REQUIRE(r < R, CHK_ERR_INVALID, "query file id out of range");
+        if (shard[i] < S && ix.shard_n[shard[i]]) q.file_n[r]++;
     }
-    CHK_REQUIRE(2 * n < (1ull << 32) - 1, CHK_ERR_INVALID, "query batch too large (2n must fit u32)");
-    hipStream_t st = q.idx->stream;
+    // method per file: the sparse path enumerates pairs through the layered
+    // table, the dense path streams the whole DB; a file goes sparse when it
+    // is small against the DB
+    const uint64_t smax = ix.tune.sparse_max ? ix.tune.sparse_max : std::max<uint64_t>(1, ix.n / 1024);
+    q.file_dense.assign(R, 0);
+    q.dense_rows.clear();
+    std::vector<uint32_t> drank(R, 0xFFFFFFFFu);
+    for (uint32_t r = 0; r < R; r++) {
+        bool dense = ix.tune.method == 1 || (ix.tune.method == 0 && q.file_n[r] > smax);
+        if (q.file_n[r] == 0) dense = false;
+        q.file_dense[r] = dense ? 1 : 0;
+        if (dense) {
+            drank[r] = (uint32_t)q.dense_rows.size();
+            q.dense_rows.push_back(r);
+        }
+    }
+    const uint32_t ND = (uint32_t)q.dense_rows.size();
+    // placement: dense files in file order, then sparse files in file order
+    std::vector<uint64_t> pos(R + 1, 0);
+    {
+        uint64_t p = 0;
+        for (uint32_t r = 0; r < R; r++)
+            if (q.file_dense[r]) { pos[r] = p; p += q.file_n[r]; }
+        q.n_dense = p;
+        for (uint32_t r = 0; r < R; r++)
+            if (!q.file_dense[r]) { pos[r] = p; p += q.file_n[r]; }
+        q.n = p;
+    }
+    CHK_REQUIRE(2 * q.n_dense < (1ull << 32) - 1, CHK_ERR_INVALID,
+                "query batch too large (2 x dense intervals must fit u32)");
+    CHK_REQUIRE(q.n < (1ull << 32), CHK_ERR_INVALID, "query batch exceeds 2^32 intervals");
+    const uint64_t n = q.n;
+    std::vector<uint32_t> hs(n), hb(n), he(n), hr(n), hg(q.n_dense);
+    q.seg_n.assign((size_t)ND * S, 0);
+    {
+        std::vector<uint64_t> fill(pos.begin(), pos.begin() + R);
+        for (uint64_t i = 0; i < n_in; i++) {
+            const uint32_t s = shard[i];
+            if (s >= S || !ix.shard_n[s]) continue;
+            const uint32_t r = qfile ? qfile[i] : 0u;
+            const uint64_t k = fill[r]++;
+            hs[k] = s;
+            hb[k] = start[i];
+            he[k] = end[i];
+            hr[k] = r;
+            if (q.file_dense[r]) {
+                hg[k] = drank[r] * S + s;
+                q.seg_n[(size_t)drank[r] * S + s]++;
+            }
+        }
+    }
+    hipStream_t st = ix.stream;
     q.shard.reserve(n);
     q.start.reserve(n);
     q.end.reserve(n);
+    q.row.reserve(n);
+    q.segid.reserve(q.n_dense);
     if (n) {
-        CHK_HIP(hipMemcpyAsync(q.shard.p, shard, n * 4, hipMemcpyHostToDevice, st));
-        CHK_HIP(hipMemcpyAsync(q.start.p, start, n * 4, hipMemcpyHostToDevice, st));
-        CHK_HIP(hipMemcpyAsync(q.end.p, end, n * 4, hipMemcpyHostToDevice, st));
+        CHK_HIP(hipMemcpyAsync(q.shard.p, hs.data(), n * 4, hipMemcpyHostToDevice, st));
+        CHK_HIP(hipMemcpyAsync(q.start.p, hb.data(), n * 4, hipMemcpyHostToDevice, st));
+        CHK_HIP(hipMemcpyAsync(q.end.p, he.data(), n * 4, hipMemcpyHostToDevice, st));
+        CHK_HIP(hipMemcpyAsync(q.row.p, hr.data(), n * 4, hipMemcpyHostToDevice, st));
     }
-    if (qfile && q.nfiles > 1 && n) {
-        q.qfile.reserve(n);
-        CHK_HIP(hipMemcpyAsync(q.qfile.p, qfile, n * 4, hipMemcpyHostToDevice, st));
-    }
-    CHK_HIP(hipStreamSynchronize(st));
-}
-
-void query_prepare(Query& q) {
-    Index& ix = *q.idx;
-    hipStream_t st = ix.stream;
-    const size_t m = 2 * q.n;
-    const int shard_bits = (int)bits_for(ix.nshards ? ix.nshards - 1 : 0) + 0;
-    const int sb = shard_bits ? shard_bits : 1;
-    const uint32_t nseg = q.nfiles * ix.nshards;
+    if (q.n_dense)
+        CHK_HIP(hipMemcpyAsync(q.segid.p, hg.data(), q.n_dense * 4, hipMemcpyHostToDevice, st));
+    const size_t m = 2 * q.n_dense;
     q.keys.reserve(m);
     q.key32.reserve(m + 1);
     q.rs.reserve(m + 1);
     q.w.reserve(m + 1);
-    q.seg.reserve(2 * (size_t)nseg);
-    hipEvent_t t0 = ix.prof.begin(st);
-    CHK_HIP(hipMemsetAsync(q.seg.p, 0, 2 * (size_t)nseg * 4, st));
-    CHK_HIP(hipMemsetAsync(q.key32.p + m, 0xFF, sizeof(uint32_t), st));  // sentinel
-    if (m) {
-        hipLaunchKernelGGL(make_event_keys, dim3(nblk(q.n)), dim3(NT), 0, st, q.shard.p, q.start.p,
-                           q.end.p, q.nfiles > 1 ? q.qfile.p : nullptr, (size_t)q.n, ix.nshards, sb,
-                           q.keys.p);
-        uint32_t nbits = 32 + sb + bits_for(q.nfiles);
-        radix_sort_u64(q.keys.p, nullptr, m, nbits, ix.scratch, st);
-        hipLaunchKernelGGL(segment_and_values, dim3(nblk(m)), dim3(NT), 0, st, q.keys.p, m,
-                           ix.nshards, sb, nseg, q.seg.p, q.rs.p, q.w.p, q.key32.p);
-        CHK_HIP(hipGetLastError());
-    }
-    exclusive_scan_u32(q.rs.p, m, ix.scratch, st);
-    exclusive_scan_i64(q.w.p, m, q.w_sums, st);
-    ix.prof.end("qprep", t0, st);
-    q.seg_host.resize(2 * (size_t)nseg);
-    CHK_HIP(hipMemcpyAsync(q.seg_host.data(), q.seg.p, 2 * (size_t)nseg * 4, hipMemcpyDeviceToHost, st));
     CHK_HIP(hipStreamSynchronize(st));
+    q.plan_version = 0xFFFFFFFFu;
+    query_plan(q);
+}
+
+// Sort the dense files' events and build the prefix sums (enqueued on st).
+void query_events(Query& q, hipStream_t st) {
+    Index& ix = *q.idx;
+    const size_t m = 2 * q.n_dense;
+    if (m) {
+        hipLaunchKernelGGL(make_event_keys, dim3(nblk(q.n_dense)), dim3(NT), 0, st, q.segid.p,
+                           q.start.p, q.end.p, (size_t)q.n_dense, q.keys.p);
+        const uint32_t nseg = (uint32_t)q.dense_rows.size() * ix.nshards;
+        radix_sort_u64(q.keys.p, nullptr, m, 32 + bits_for(nseg - 1), q.scratch, st);
+    }
+    hipLaunchKernelGGL(event_values, dim3(nblk(m + 1)), dim3(NT), 0, st, q.keys.p, m, q.rs.p, q.w.p,
+                       q.key32.p);
+    CHK_HIP(hipGetLastError());
+    exclusive_scan_u32(q.rs.p, m, q.scratch, st);
+    exclusive_scan_i64(q.w.p, m, q.w_sums, st);
 }
 
 }  // namespace chk

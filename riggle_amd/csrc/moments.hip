@@ -234,20 +234,27 @@ __global__ __launch_bounds__(NT) void fft_cols_inv(Rows r, uint32_t N1, uint32_t
 
 // ------------------------------------------------------------ binning
 
-template <typename FidT>
-__global__ void impulses(const int32_t* coord, const FidT* fid, uint64_t cnt, int32_t sign,
-                         const int32_t* slot_of_fid, uint32_t n, uint32_t ldg, int32_t* D) {
+// +coverage impulses at each start and -impulses at each end of the
+// intervals of one shard (layered table order), split over the two bins a
+// coordinate straddles: bin x/100 gets 100 - x%100, the next bin x%100.
+__global__ void impulses(const int32_t* start, const int32_t* end, const uint32_t* fid,
+                         uint64_t cnt, const int32_t* slot_of_fid, uint32_t n, uint32_t ldg,
+                         int32_t* D) {
     const uint32_t max_bp = n * 100u;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const int32_t s = slot_of_fid[(uint32_t)fid[i]];
+        const int32_t s = slot_of_fid[fid[i]];
         if (s < 0) continue;
-        uint32_t x = (uint32_t)coord[i];
-        x = x < max_bp ? x : max_bp;
-        const uint32_t bin = x / 100u, rem = x % 100u;
         int32_t* row = D + (size_t)s * ldg;
-        if (bin < n) atomicAdd(&row[bin], sign * (int32_t)(100u - rem));
-        if (rem > 0 && bin + 1 < n) atomicAdd(&row[bin + 1], sign * (int32_t)rem);
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            uint32_t x = (uint32_t)(e ? end[i] : start[i]);
+            const int32_t sign = e ? -1 : 1;
+            x = x < max_bp ? x : max_bp;
+            const uint32_t bin = x / 100u, rem = x % 100u;
+            if (bin < n) atomicAdd(&row[bin], sign * (int32_t)(100u - rem));
+            if (rem > 0 && bin + 1 < n) atomicAdd(&row[bin + 1], sign * (int32_t)rem);
+        }
     }
 }
 
@@ -579,7 +586,9 @@ void moments_build(Index& ix) {
         sC2.reserve((size_t)B * nl);
         sE1.reserve((size_t)B * nl);
         sQ.reserve((size_t)B * nl);
-        const uint64_t soff = ix.shard_off[s], scnt = ix.shard_off[s + 1] - ix.shard_off[s];
+        const uint32_t NL = ix.nlayers;
+        const LayerSeg& Ll = ix.lseg[(size_t)s * NL + NL - 1];
+        const uint64_t soff = ix.lseg[(size_t)s * NL].off, scnt = Ll.off + Ll.n - soff;
         for (size_t f0 = 0; f0 < files.size(); f0 += B) {
             const uint32_t nb = (uint32_t)std::min<size_t>(B, files.size() - f0);
             std::vector<int64_t> oo(nb);
@@ -591,19 +600,10 @@ void moments_build(Index& ix) {
             CHK_HIP(hipMemcpyAsync(out_off.p, oo.data(), nb * 8, hipMemcpyHostToDevice, st));
             CHK_HIP(hipMemsetAsync(D.p, 0, (size_t)nb * ldg * 4, st));
             const unsigned grid = (unsigned)std::min<uint64_t>((scnt + NT - 1) / NT, (uint64_t)ix.ncu * 16);
-            if (grid) {
-                if (ix.fid16) {
-                    hipLaunchKernelGGL(impulses<uint16_t>, dim3(grid), dim3(NT), 0, st, ix.s_coord.p + soff,
-                                       ix.s_fid16.p + soff, scnt, 1, slot_of_fid.p, n, ldg, D.p);
-                    hipLaunchKernelGGL(impulses<uint16_t>, dim3(grid), dim3(NT), 0, st, ix.e_coord.p + soff,
-                                       ix.e_fid16.p + soff, scnt, -1, slot_of_fid.p, n, ldg, D.p);
-                } else {
-                    hipLaunchKernelGGL(impulses<uint32_t>, dim3(grid), dim3(NT), 0, st, ix.s_coord.p + soff,
-                                       ix.s_fid.p + soff, scnt, 1, slot_of_fid.p, n, ldg, D.p);
-                    hipLaunchKernelGGL(impulses<uint32_t>, dim3(grid), dim3(NT), 0, st, ix.e_coord.p + soff,
-                                       ix.e_fid.p + soff, scnt, -1, slot_of_fid.p, n, ldg, D.p);
-                }
-            }
+            if (grid)
+                hipLaunchKernelGGL(impulses, dim3(grid), dim3(NT), 0, st, ix.l_start.p + soff,
+                                   ix.l_end.p + soff, ix.l_fid.p + soff, scnt, slot_of_fid.p, n,
+                                   ldg, D.p);
             hipLaunchKernelGGL(coverage_rows, dim3(nb), dim3(NT), 0, st, D.p, ldg, n, cen.p, sumg.p);
             CHK_HIP(hipGetLastError());
             Rows r{D.p, ldg, cen.p, X.p, A.p, n, (uint32_t)N, logN, tw.p};

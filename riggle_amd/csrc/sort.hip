@@ -105,10 +105,16 @@ __global__ __launch_bounds__(NT) void scan_tile_apply(T* data, size_t n, const T
 }
 
 template <typename T>
+__global__ void set_zero_one(T* data) {
+    if (threadIdx.x == 0) data[0] = 0;
+}
+
+template <typename T>
 void exclusive_scan_impl(T* data, size_t n, T* sums, hipStream_t st) {
     size_t nb = (n + SC_TILE - 1) / SC_TILE;
-    if (nb == 0) {
-        CHK_HIP(hipMemsetAsync(data, 0, sizeof(T), st));
+    if (nb == 0) {  // a kernel, not a memset: the sequence may be graph-captured
+        hipLaunchKernelGGL(set_zero_one<T>, dim3(1), dim3(64), 0, st, data);
+        CHK_HIP(hipGetLastError());
         return;
     }
     hipLaunchKernelGGL(scan_tile_sums<T>, dim3((unsigned)nb), dim3(NT), 0, st, data, n, sums);

@@ -1,7 +1,7 @@
-// sweep.hip -- merge-path overlap sweep (count + overlap bp, exact u64).
+// sweep.hip -- overlap sweep (count + overlap bp, exact u64), dense and sparse.
 //
-// For one query file Q and one DB interval d on a shard (pair-free form of the
-// reference's per-pair scan, src/sweep/query.rs:148-157):
+// Dense path.  For one query file Q and one DB interval d on a shard (pair-free
+// form of the reference's per-pair scan, src/sweep/query.rs:148-157):
 //   count(Q, d) = #{q : q.s < d.e} - #{q : q.e <= d.s}
 //   bp(Q, d)    = CQ(d.e) - CQ(d.s),   CQ(x) = sum_q |[q.s, q.e) n [0, x)|
 // With the query's sorted event stream, a DB coordinate x of rank p (events
@@ -13,22 +13,36 @@
 // Work is cut into equal tiles of TILE merged items (DB elements + query
 // events) along the merge path of each (DB stream, query segment) job, so a
 // tile costs the same wherever the query's events cluster.  Persistent blocks
-// own contiguous tile ranges and software-pipeline them: while tile t is
-// ranked and reduced, the DB elements and query events of tile t+1 are already
-// in flight into registers.  Each DB element is ranked by binary search among
-// the tile's events staged in LDS, and reduced per DB file in an LDS histogram
-// that is flushed with one u64 atomic per touched cell when the block moves to
-// another query file or ends.
-#include "engine.hpp"
-
+// take chunks of consecutive tiles from a device work queue and
+// software-pipeline them: while tile t is ranked and reduced, the DB elements
+// and query events of tile t+1 are already in flight into registers.  A wave
+// owns 64 * ITEMS consecutive DB elements of a tile; it holds the 64 events
+// following its first element in registers and ranks each 64-element row with
+// two ballots (events below the row's first and last key): rows without an
+// event inside share one rank, the others count the few events inside with
+// cross-lane reads.  Contributions are reduced per DB file in an LDS
+// histogram that is flushed with one u64 atomic per touched cell when the
+// block moves to another (query file, file group) or ends.
+//
+// Sparse path.  A small query file is swept the reference's way
+// (src/sweep/query.rs:84-117): for each query interval and each size layer,
+// a jump-table lookup finds the first DB start that can overlap it and a
+// wave scans the layer's starts up to q.end, adding every overlapping pair.
+// Its cost follows the query's pairs instead of the DB's size.
 #include <algorithm>
 
+#include "engine.hpp"
+
 namespace chk {
+
+void query_events(Query& q, hipStream_t st);
+
 namespace {
 
 constexpr int NT = 256;
-constexpr int ITEMS = 8;
-constexpr uint32_t TILE = NT * ITEMS;
+constexpr uint32_t QS = 512;    // events staged per tile on the fast path (nq + 1 <= QS)
+constexpr uint32_t MAXC = 64;   // tiles per work item at most
+constexpr uint32_t SPI = 256;   // query intervals per sparse work item (default)
 
 __device__ inline uint32_t db_key(int32_t x) { return 2u * (uint32_t)x + 1u; }
 
@@ -36,9 +50,30 @@ __device__ inline uint32_t db_key(int32_t x) { return 2u * (uint32_t)x + 1u; }
 // tiles stay in flight across it (__syncthreads would also wait for them).
 __device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+__device__ inline uint32_t ufl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t ufl64(int64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ inline uint32_t rdl(uint32_t v, uint32_t l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline int64_t rdl64(int64_t v, uint32_t l) {
+    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, l);
+    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((uint64_t)v >> 32), l);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ inline uint32_t popc(uint64_t m) { return (uint32_t)__builtin_popcountll(m); }
+
+constexpr int BUF_DWORD3 = 0x00020000;  // raw buffer, gfx9-family descriptor word 3
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                             BUF_DWORD3);
+}
+
 // Merge-path split of every tile boundary: part[b] = # DB elements among the
 // first d merged items of its job (DB element before event iff dkey <= qkey).
-__global__ void partition_kernel(const Job* jobs, uint32_t njobs, uint32_t nbound,
+__global__ void partition_kernel(const Job* jobs, uint32_t njobs, uint32_t nbound, uint32_t tile,
                                  const int32_t* s_coord, const int32_t* e_coord,
                                  const uint32_t* qkeys, uint32_t* part) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -52,7 +87,7 @@ __global__ void partition_kernel(const Job* jobs, uint32_t njobs, uint32_t nboun
     const int32_t* D = (J.is_end ? e_coord : s_coord) + J.db_off;
     const uint32_t* Q = qkeys + J.q_off;
     uint64_t total = (uint64_t)J.n_db + J.m;
-    uint64_t d = std::min<uint64_t>((uint64_t)(b - J.part0) * TILE, total);
+    uint64_t d = std::min<uint64_t>((uint64_t)(b - J.part0) * tile, total);
     uint64_t a = d > J.m ? d - J.m : 0, z = std::min<uint64_t>(d, J.n_db);
     while (a < z) {
         uint64_t mid = (a + z) >> 1;
@@ -63,35 +98,33 @@ __global__ void partition_kernel(const Job* jobs, uint32_t njobs, uint32_t nboun
 
 struct SweepArgs {
     const Job* jobs;
-    uint32_t njobs, ntiles, tiles_per_block, F, pwin;
+    const uint32_t* chunk_job;  // nchunks + 1
     const uint32_t* part;
+    uint32_t ntiles, nchunks, chunk, F;
     const int32_t* s_coord;
     const int32_t* e_coord;
-    const void* s_fid;
-    const void* e_fid;
+    const uint16_t* s_lfid;
+    const uint16_t* e_lfid;
     const uint32_t* qkey;  // 2n+1 entries (sentinel at the end)
     const uint32_t* qrs;   // 2n+1
     const int64_t* qw;     // 2n+1
+    uint32_t* queue;
     unsigned long long* counts;
     unsigned long long* bp;
 };
 
-constexpr uint32_t QS = NT;     // query events staged per tile on the fast path
-constexpr uint32_t MAXJ = 64;   // jobs a block may span (the host sizes blocks so)
-constexpr uint32_t PWIN = 1024; // partition entries held in LDS at a time
-
-// A job as a block sees it, with its segment's prefix bases.
-struct LJob {
-    Job J;
-    uint32_t tile_end;
-    uint32_t rs0;
-    int64_t w0;
+// One tile as the block sees it.
+struct TMeta {
+    uint64_t dbase;  // first DB element of the tile (stream index)
+    uint64_t cell;   // output cell of group-local file 0
+    int64_t w0;      // W at the job's first event
+    uint32_t nd, nq, qb, j0, rs0, is_end, nf, pad;
 };
 
-// Uniform description of one tile.
 struct Meta {
-    uint64_t dbase, dlast;  // first element of the tile / last element of the stream
-    uint32_t qb, nd, nq, j0, qfile, is_end, jl;
+    uint64_t dbase, cell;
+    int64_t w0;
+    uint32_t nd, nq, qb, j0, rs0, is_end, nf;
 };
 
 // Registers holding one tile in flight.  Buffer loads bound every read by
@@ -99,221 +132,180 @@ struct Meta {
 // per-lane clamping and are issued unconditionally: the compiler then counts
 // outstanding loads statically and keeps the next tile's loads in flight
 // while this one is reduced.
-template <typename FidT>
+template <int ITEMS>
 struct Stage {
     int32_t x[ITEMS];
-    FidT f[ITEMS];
-    uint32_t key, rs;
-    int64_t w;
+    uint32_t f[ITEMS];
+    uint32_t key[2], rs[2];
+    int64_t w[2];
 };
 
-constexpr int BUF_DWORD3 = 0x00020000;  // raw buffer, gfx9-family descriptor word 3
-
-__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
-                                             BUF_DWORD3);
-}
-
-__device__ inline uint32_t ufl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ inline int64_t ufl64(int64_t v) {
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
-    return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-template <typename FidT>
-__device__ inline void stage_load(const SweepArgs& a, const Meta& m, Stage<FidT>& s) {
-    // query events first: staging them into LDS then waits only for these
-    // three loads, leaving the DB loads in flight across the barrier
-    const uint32_t nst = m.nq + 1 < QS ? m.nq + 1 : QS;  // ranks lie in [0, nq]
-    s.key = __builtin_amdgcn_raw_buffer_load_b32(make_rsrc(a.qkey + m.qb, nst * 4u),
-                                                 threadIdx.x * 4u, 0, 0);
-    s.rs = __builtin_amdgcn_raw_buffer_load_b32(make_rsrc(a.qrs + m.qb, nst * 4u),
-                                                threadIdx.x * 4u, 0, 0);
-    s.w = (int64_t)__builtin_amdgcn_raw_buffer_load_b64(make_rsrc(a.qw + m.qb, nst * 8u),
-                                                        threadIdx.x * 8u, 0, 0);
-    const int32_t* coord = (m.is_end ? a.e_coord : a.s_coord) + m.dbase;
-    const FidT* fid = reinterpret_cast<const FidT*>(m.is_end ? a.e_fid : a.s_fid) + m.dbase;
-    const __amdgpu_buffer_rsrc_t rc = make_rsrc(coord, m.nd * 4u);
-    const __amdgpu_buffer_rsrc_t rf = make_rsrc(fid, m.nd * (uint32_t)sizeof(FidT));
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const uint32_t i = threadIdx.x + k * NT;
-        s.x[k] = __builtin_amdgcn_raw_buffer_load_b32(rc, i * 4u, 0, 0);
-        if constexpr (sizeof(FidT) == 2)
-            s.f[k] = __builtin_amdgcn_raw_buffer_load_b16(rf, i * 2u, 0, 0);
-        else
-            s.f[k] = __builtin_amdgcn_raw_buffer_load_b32(rf, i * 4u, 0, 0);
-    }
-}
-
-template <typename FidT>
+template <int ITEMS>
 __global__ __launch_bounds__(NT) void sweep_kernel(SweepArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const uint32_t F = a.F;
-    unsigned long long* hc = reinterpret_cast<unsigned long long*>(smem);
-    unsigned long long* hb = hc + F;
-    uint32_t* lkey = reinterpret_cast<uint32_t*>(hb + F);     // QS (16-byte aligned)
-    int64_t* lw = reinterpret_cast<int64_t*>(lkey + QS);      // QS
-    LJob* ljob = reinterpret_cast<LJob*>(lw + QS);            // MAXJ
-    uint32_t* lrs = reinterpret_cast<uint32_t*>(ljob + MAXJ); // QS
-    uint32_t* lpart = lrs + QS;                               // PWIN
+    constexpr uint32_t IPW = 64u * ITEMS;  // DB elements per wave per tile
+    constexpr uint32_t TILE = NT * ITEMS;
+    __shared__ unsigned long long hc[FG], hb[FG];
+    __shared__ uint32_t lkey[QS], lrs[QS];
+    __shared__ int64_t lw[QS];
+    __shared__ TMeta tm[MAXC];
+    __shared__ Job lj[MAXC + 1];
+    __shared__ uint32_t s_chunk;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index (scalar)
+    const uint32_t wv = ufl(tid >> 6);  // wave index (scalar)
+    const uint32_t F = a.F;
+    for (uint32_t f = tid; f < FG; f += NT) hc[f] = hb[f] = 0ull;
+    uint64_t cur_cell = ~0ull;
+    uint32_t cur_nf = 0;
 
-    const uint32_t t_begin = blockIdx.x * a.tiles_per_block;
-    const uint32_t t_end = min(a.ntiles, t_begin + a.tiles_per_block);
-    if (t_begin >= t_end) return;
-    for (uint32_t f = tid; f < F; f += NT) hc[f] = hb[f] = 0ull;
-
-    // Jobs spanned by this block's tiles.
-    uint32_t jfirst, jlast;
-    {
-        uint32_t lo = 0, hi = a.njobs;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (a.jobs[mid].tile0 <= t_begin) lo = mid; else hi = mid;
-        }
-        jfirst = lo;
-        hi = a.njobs;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (a.jobs[mid].tile0 <= t_end - 1) lo = mid; else hi = mid;
-        }
-        jlast = lo;
-    }
-    const uint32_t nj = jlast - jfirst + 1;  // <= MAXJ
-    for (uint32_t j = tid; j < nj; j += NT) {
-        LJob L;
-        L.J = a.jobs[jfirst + j];
-        L.tile_end = jfirst + j + 1 < a.njobs ? a.jobs[jfirst + j + 1].tile0 : a.ntiles;
-        L.rs0 = a.qrs[L.J.q_off];
-        L.w0 = a.qw[L.J.q_off];
-        ljob[j] = L;
-    }
-    uint32_t pbase = 0xFFFFFFFFu;
-    uint32_t jl = 0;
-    uint32_t cur_q = 0xFFFFFFFFu;
-    lds_barrier();
-
-    auto refill = [&](uint32_t base) {
-        lds_barrier();
-        const uint32_t bound = a.ntiles + a.njobs;  // total partition entries
-        for (uint32_t i = tid; i < a.pwin; i += NT)
-            if (base + i < bound) lpart[i] = a.part[base + i];
-        lds_barrier();
-        pbase = base;
-    };
-
-    // Uniform description of tile t (values read from LDS are made scalar).
-    auto meta = [&](uint32_t t) {
-        while (t >= ufl(ljob[jl].tile_end)) jl++;
-        const uint32_t tile0 = ufl(ljob[jl].J.tile0), part0 = ufl(ljob[jl].J.part0);
-        const uint32_t n_db = ufl(ljob[jl].J.n_db), mq = ufl(ljob[jl].J.m);
-        const uint32_t q_off = ufl(ljob[jl].J.q_off);
-        const uint64_t db_off = (uint64_t)ufl64((int64_t)ljob[jl].J.db_off);
-        const uint32_t k = t - tile0;
-        const uint32_t pi = part0 + k;
-        if (pbase == 0xFFFFFFFFu || pi < pbase || pi + 1 >= pbase + a.pwin) refill(pi);
-        const uint32_t i0 = ufl(lpart[pi - pbase]), i1 = ufl(lpart[pi + 1 - pbase]);
-        const uint64_t total = (uint64_t)n_db + mq;
-        const uint64_t d0 = (uint64_t)k * TILE;
-        const uint64_t d1 = d0 + TILE < total ? d0 + TILE : total;
-        Meta m;
-        m.dbase = db_off + i0;
-        m.dlast = db_off + n_db - 1;
-        m.nd = i1 - i0;
-        m.j0 = (uint32_t)(d0 - i0);
-        m.nq = (uint32_t)(d1 - i1) - m.j0;
-        m.qb = q_off + m.j0;
-        m.qfile = ufl(ljob[jl].J.qfile);
-        m.is_end = ufl(ljob[jl].J.is_end);
-        m.jl = jl;
-        return m;
-    };
-
-    auto flush = [&](uint32_t q) {
-        for (uint32_t f = tid; f < F; f += NT) {
-            unsigned long long cv = hc[f], bv = hb[f];
+    auto flush = [&]() {
+        if (cur_cell == ~0ull) return;
+        for (uint32_t f = tid; f < cur_nf; f += NT) {
+            const unsigned long long cv = hc[f], bv = hb[f];
             if (cv | bv) {
-                atomicAdd(&a.counts[(size_t)q * F + f], cv);
-                atomicAdd(&a.bp[(size_t)q * F + f], bv);
+                atomicAdd(&a.counts[cur_cell + f], cv);
+                atomicAdd(&a.bp[cur_cell + f], bv);
                 hc[f] = hb[f] = 0ull;
             }
         }
     };
 
-    auto stage_store = [&](const Stage<FidT>& s, const Meta& m) {
-        if (m.qfile != cur_q) {
-            if (cur_q != 0xFFFFFFFFu) flush(cur_q);
-            cur_q = m.qfile;
+    auto meta = [&](uint32_t k) {
+        Meta m;
+        m.dbase = (uint64_t)ufl64((int64_t)tm[k].dbase);
+        m.cell = (uint64_t)ufl64((int64_t)tm[k].cell);
+        m.w0 = ufl64(tm[k].w0);
+        m.nd = ufl(tm[k].nd);
+        m.nq = ufl(tm[k].nq);
+        m.qb = ufl(tm[k].qb);
+        m.j0 = ufl(tm[k].j0);
+        m.rs0 = ufl(tm[k].rs0);
+        m.is_end = ufl(tm[k].is_end);
+        m.nf = ufl(tm[k].nf);
+        return m;
+    };
+
+    auto stage_load = [&](const Meta& m, Stage<ITEMS>& s) {
+        // query events first: staging them into LDS then waits only for these
+        // loads, leaving the DB loads in flight across the barrier
+        const uint32_t nst = m.nq + 1 < QS ? m.nq + 1 : QS;  // ranks lie in [0, nq]
+        const auto rk = make_rsrc(a.qkey + m.qb, nst * 4u);
+        const auto rr = make_rsrc(a.qrs + m.qb, nst * 4u);
+        const auto rw = make_rsrc(a.qw + m.qb, nst * 8u);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t i = tid + h * NT;
+            s.key[h] = __builtin_amdgcn_raw_buffer_load_b32(rk, i * 4u, 0, 0);
+            s.rs[h] = __builtin_amdgcn_raw_buffer_load_b32(rr, i * 4u, 0, 0);
+            s.w[h] = (int64_t)__builtin_amdgcn_raw_buffer_load_b64(rw, i * 8u, 0, 0);
         }
-        // every slot is written: keys past nq hold a sentinel above any DB key
-        lkey[tid] = tid < m.nq ? s.key : 0xFFFFFFFFu;
-        if (tid <= m.nq) {
-            lrs[tid] = s.rs - ljob[m.jl].rs0;
-            lw[tid] = s.w - ljob[m.jl].w0;
+        const int32_t* coord = (m.is_end ? a.e_coord : a.s_coord) + m.dbase;
+        const uint16_t* fid = (m.is_end ? a.e_lfid : a.s_lfid) + m.dbase;
+        const auto rc = make_rsrc(coord, m.nd * 4u);
+        const auto rf = make_rsrc(fid, m.nd * 2u);
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t i = wv * IPW + k * 64u + lane;
+            s.x[k] = __builtin_amdgcn_raw_buffer_load_b32(rc, i * 4u, 0, 0);
+            s.f[k] = __builtin_amdgcn_raw_buffer_load_b16(rf, i * 2u, 0, 0);
         }
     };
 
-    // Per-element contribution: rank p (events before the element), prefix
-    // values (rs, w) at p.  E stream: (+rs, x*depth - w); S stream:
-    // (-(p - rs), -(x*depth - w)); depth = 2 rs - p.
-    auto add = [&](const Meta& m, int32_t x, uint32_t f, uint32_t p, uint32_t rsv, int64_t wvv) {
-        const int32_t depth = 2 * (int32_t)rsv - (int32_t)(m.j0 + p);
-        const int64_t cq = (int64_t)x * (int64_t)depth - wvv;
-        unsigned long long cv, bv;
-        if (m.is_end) {
-            cv = (unsigned long long)(int64_t)rsv;
-            bv = (unsigned long long)cq;
-        } else {
-            cv = (unsigned long long)(-(int64_t)(m.j0 + p - rsv));
-            bv = (unsigned long long)(-cq);
+    auto stage_store = [&](const Stage<ITEMS>& s, const Meta& m) {
+        if (m.cell != cur_cell) {
+            flush();
+            cur_cell = m.cell;
+            cur_nf = m.nf;
         }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t i = tid + h * NT;
+            if (i <= m.nq) {  // i < QS always
+                lkey[i] = i < m.nq ? s.key[h] : 0xFFFFFFFFu;
+                lrs[i] = s.rs[h] - m.rs0;
+                lw[i] = s.w[h] - m.w0;
+            }
+        }
+    };
+
+    // Per-element contribution at job rank P with prefix values (rs, w) at P.
+    // E stream: (+rs, x*depth - w); S stream: (rs - P, -(x*depth - w));
+    // depth = 2 rs - P.
+    auto add = [&](const Meta& m, int32_t x, uint32_t f, uint32_t P, uint32_t rsv, int64_t wq) {
+        const int32_t depth = 2 * (int32_t)rsv - (int32_t)P;
+        const int32_t dep = m.is_end ? depth : -depth;
+        const int64_t wn = m.is_end ? -wq : wq;
+        const unsigned long long cv =
+            (unsigned long long)(m.is_end ? (int64_t)rsv : (int64_t)rsv - (int64_t)P);
+        const unsigned long long bv = (unsigned long long)((int64_t)x * (int64_t)dep + wn);
         atomicAdd(&hc[f], cv);
         atomicAdd(&hb[f], bv);
     };
 
-    auto compute = [&](const Stage<FidT>& s, const Meta& m) {
+    // rank of key dk among the staged events: two-level ballot search
+    auto rank_lds = [&](uint32_t dk, uint32_t nq) {
+        const uint32_t si = lane * 8u;
+        const uint32_t sv = si < nq ? lkey[si] : 0xFFFFFFFFu;
+        const uint32_t c1 = popc(__ballot(sv < dk));
+        const uint32_t base = c1 ? (c1 - 1u) * 8u : 0u;
+        const uint32_t li = base + lane;
+        const uint32_t lv = (lane < 8u && li < nq) ? lkey[li] : 0xFFFFFFFFu;
+        return base + popc(__ballot(lv < dk));
+    };
+
+    auto compute = [&](const Stage<ITEMS>& s, const Meta& m) {
         const uint32_t nq = m.nq, nd = m.nd;
-        if (nq < QS) {
-            // Wave wv owns sub-rows of 64 consecutive elements (k*NT + 64 wv +
-            // lane).  A sub-row with no event key inside its key span has one
-            // rank for all lanes, found with a uniform cursor over the sorted
-            // keys; otherwise lanes count the few keys inside the span.
-            uint32_t j = 0;
+        const uint32_t wbase = wv * IPW;
+        if (wbase >= nd) return;
+        if (nq + 1 <= QS) {
+            uint32_t jlo = 0, kv = 0, rv = 0;
+            int64_t wvv = 0;
+            auto window = [&](uint32_t dk) {
+                jlo = ufl(rank_lds(dk, nq));
+                const uint32_t wi = jlo + lane;
+                kv = wi < nq ? lkey[wi] : 0xFFFFFFFFu;
+                rv = wi <= nq ? lrs[wi] : 0u;
+                wvv = wi <= nq ? lw[wi] : 0;
+            };
+            window(ufl(db_key(s.x[0])));
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
-                const uint32_t base = k * NT + 64u * wv;
+                const uint32_t base = wbase + k * 64u;
                 if (base < nd) {
                     const uint32_t nv = nd - base < 64u ? nd - base : 64u;
                     const uint32_t dk = db_key(s.x[k]);
-                    const uint32_t dkf = ufl(dk);
-                    const uint32_t dkl = __builtin_amdgcn_readlane(dk, nv - 1);
-                    uint32_t kj = ufl(lkey[j]);
-                    while (kj < dkf) kj = ufl(lkey[++j]);
-                    if (kj >= dkl) {
-                        const uint32_t rsv = ufl(lrs[j]);
-                        const int64_t wq = ufl64(lw[j]);
-                        if (lane < nv) add(m, s.x[k], (uint32_t)s.f[k], j, rsv, wq);
+                    const uint32_t dkf = ufl(dk), dkl = rdl(dk, nv - 1u);
+                    uint32_t ea = popc(__ballot(kv < dkf)), eb = popc(__ballot(kv < dkl));
+                    if (eb == 64u) {  // window exhausted: re-anchor at this row
+                        window(dkf);
+                        ea = popc(__ballot(kv < dkf));
+                        eb = popc(__ballot(kv < dkl));
+                    }
+                    if (ea == eb) {
+                        const uint32_t rsv = rdl(rv, ea);
+                        const int64_t wq = rdl64(wvv, ea);
+                        if (lane < nv) add(m, s.x[k], s.f[k], m.j0 + jlo + ea, rsv, wq);
+                    } else if (eb < 64u) {
+                        uint32_t c = ea;
+                        for (uint32_t t = ea; t < eb; t++) c += rdl(kv, t) < dk ? 1u : 0u;
+                        const uint32_t p = jlo + c;
+                        if (lane < nv) add(m, s.x[k], s.f[k], m.j0 + p, lrs[p], lw[p]);
                     } else {
-                        uint32_t p = j, jj = j, kk = kj;
-                        while (kk < dkl) {
-                            p += kk < dk ? 1u : 0u;
-                            kk = ufl(lkey[++jj]);
+                        // more than 63 events inside one row: per-lane search
+                        uint32_t lo = jlo + ea, hi = nq;
+                        while (lo < hi) {
+                            const uint32_t mid = (lo + hi) >> 1;
+                            if (lkey[mid] < dk) lo = mid + 1; else hi = mid;
                         }
-                        if (lane < nv) add(m, s.x[k], (uint32_t)s.f[k], p, lrs[p], lw[p]);
+                        if (lane < nv) add(m, s.x[k], s.f[k], m.j0 + lo, lrs[lo], lw[lo]);
                     }
                 }
             }
         } else {
             // event-dense tile: per-lane binary search over the keys in memory
-            const uint32_t rs_base = ljob[m.jl].rs0;
-            const int64_t w_base = ljob[m.jl].w0;
 #pragma unroll
             for (int k = 0; k < ITEMS; k++) {
-                const uint32_t i = tid + k * NT;
+                const uint32_t i = wbase + k * 64u + lane;
                 if (i < nd) {
                     const uint32_t dk = db_key(s.x[k]);
                     uint32_t lo = 0, hi = nq;
@@ -321,122 +313,321 @@ __global__ __launch_bounds__(NT) void sweep_kernel(SweepArgs a) {
                         const uint32_t mid = (lo + hi) >> 1;
                         if (a.qkey[m.qb + mid] < dk) lo = mid + 1; else hi = mid;
                     }
-                    add(m, s.x[k], (uint32_t)s.f[k], lo, a.qrs[m.qb + lo] - rs_base,
-                        a.qw[m.qb + lo] - w_base);
+                    add(m, s.x[k], s.f[k], m.j0 + lo, a.qrs[m.qb + lo] - m.rs0,
+                        a.qw[m.qb + lo] - m.w0);
                 }
             }
         }
     };
 
-    // Two register stages: the loads of one tile are issued before the other
-    // tile is staged and reduced.  Past the range end the last tile is
-    // reloaded (results unused) so every iteration issues the same loads.
-    Stage<FidT> A, B;
-    Meta mA = meta(t_begin);
-    stage_load(a, mA, A);
-    Meta mB = meta(min(t_begin + 1, t_end - 1));
-    for (uint32_t t = t_begin;; t += 2) {
-        stage_load(a, mB, B);
-        stage_store(A, mA);
-        lds_barrier();
-        compute(A, mA);
-        lds_barrier();
-        if (t + 1 >= t_end) break;
-        mA = meta(min(t + 2, t_end - 1));
-        stage_load(a, mA, A);
-        stage_store(B, mB);
-        lds_barrier();
-        compute(B, mB);
-        lds_barrier();
-        if (t + 2 >= t_end) break;
-        mB = meta(min(t + 3, t_end - 1));
+    for (;;) {
+        if (tid == 0) s_chunk = atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const uint32_t c = ufl(s_chunk);
+        if (c >= a.nchunks) break;
+        const uint32_t t0 = c * a.chunk;
+        const uint32_t nt = min(a.chunk, a.ntiles - t0);
+        const uint32_t ja = a.chunk_job[c], jn = a.chunk_job[c + 1] - ja + 1;
+        for (uint32_t j = tid; j < jn; j += NT) lj[j] = a.jobs[ja + j];
+        __syncthreads();
+        if (tid < nt) {
+            const uint32_t t = t0 + tid;
+            uint32_t k = 0;
+            while (k + 1 < jn && lj[k + 1].tile0 <= t) k++;
+            const Job J = lj[k];
+            const uint32_t kt = t - J.tile0;
+            const uint32_t pi = J.part0 + kt;
+            const uint32_t i0 = a.part[pi], i1 = a.part[pi + 1];
+            const uint64_t total = (uint64_t)J.n_db + J.m;
+            const uint64_t d0 = (uint64_t)kt * TILE;
+            const uint64_t d1 = d0 + TILE < total ? d0 + TILE : total;
+            TMeta T;
+            T.dbase = J.db_off + i0;
+            T.nd = i1 - i0;
+            T.j0 = (uint32_t)(d0 - i0);
+            T.nq = (uint32_t)(d1 - i1) - T.j0;
+            T.qb = J.q_off + T.j0;
+            T.rs0 = a.qrs[J.q_off];
+            T.w0 = a.qw[J.q_off];
+            T.cell = (uint64_t)J.row * F + (uint64_t)J.group * FG;
+            T.nf = min(FG, F - J.group * FG);
+            T.is_end = J.is_end;
+            T.pad = 0;
+            tm[tid] = T;
+        }
+        __syncthreads();
+        // two register stages: the loads of one tile are issued before the
+        // other tile is staged and reduced; past the chunk's end the last
+        // tile is reloaded (results unused) so every iteration issues the
+        // same loads
+        Stage<ITEMS> A, B;
+        Meta mA = meta(0);
+        stage_load(mA, A);
+        Meta mB = meta(min(1u, nt - 1));
+        for (uint32_t k = 0;; k += 2) {
+            stage_load(mB, B);
+            stage_store(A, mA);
+            lds_barrier();
+            compute(A, mA);
+            lds_barrier();
+            if (k + 1 >= nt) break;
+            mA = meta(min(k + 2, nt - 1));
+            stage_load(mA, A);
+            stage_store(B, mB);
+            lds_barrier();
+            compute(B, mB);
+            lds_barrier();
+            if (k + 2 >= nt) break;
+            mB = meta(min(k + 3, nt - 1));
+        }
     }
-    flush(cur_q);
+    flush();
+}
+
+struct SparseArgs {
+    const SparseItem* items;
+    const uint32_t* shard;
+    const uint32_t* start;
+    const uint32_t* end;
+    const LayerSeg* lseg;
+    uint32_t nlayers, F;
+    const int32_t* l_start;
+    const int32_t* l_end;
+    const uint32_t* l_fid;
+    const uint32_t* jt;
+    unsigned long long* counts;
+    unsigned long long* bp;
+};
+
+// One block per work item; each wave takes query intervals of the item in
+// turn and, per layer, scans the DB starts in [q.s - lookback + 1, q.e)
+// 64 at a time.  With LH the pairs are reduced in an LDS histogram of all F
+// files (flushed once per item), else added to the output directly.
+template <bool LH>
+__global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
+    extern __shared__ unsigned long long sh[];
+    unsigned long long* hc = sh;
+    unsigned long long* hb = sh + a.F;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wv = ufl(tid >> 6);
+    const SparseItem it = a.items[blockIdx.x];
+    const uint64_t rowcell = (uint64_t)it.row * a.F;
+    if (LH) {
+        for (uint32_t f = tid; f < a.F; f += NT) hc[f] = hb[f] = 0ull;
+        __syncthreads();
+    }
+    for (uint32_t i = it.begin + wv; i < it.end; i += NT / 64) {
+        const uint32_t s = ufl(a.shard[i]);
+        const int32_t qs = (int32_t)ufl(a.start[i]), qe = (int32_t)ufl(a.end[i]);
+        for (uint32_t K = 0; K < a.nlayers; K++) {
+            const LayerSeg L = a.lseg[(size_t)s * a.nlayers + K];
+            if (L.n == 0) continue;
+            const int64_t lo64 = (int64_t)qs - (int64_t)L.lookback + 1;
+            const int32_t lo = lo64 > 0 ? (int32_t)lo64 : 0;
+            const uint32_t bkt = (uint32_t)lo >> L.shift;
+            uint32_t p = bkt < L.nb ? a.jt[L.jt + bkt] : L.n;
+            while (p < L.n) {
+                const uint32_t idx = p + lane;
+                const bool valid = idx < L.n;
+                const int32_t st = valid ? a.l_start[L.off + idx] : 0x7FFFFFFF;
+                const bool before_end = valid && st < qe;
+                const uint64_t more = __ballot(before_end);
+                if (before_end && st >= lo) {
+                    const int32_t en = a.l_end[L.off + idx];
+                    if (en > qs) {
+                        const uint32_t f = a.l_fid[L.off + idx];
+                        const int64_t ov = (int64_t)min(qe, en) - (int64_t)max(qs, st);
+                        if (LH) {
+                            atomicAdd(&hc[f], 1ull);
+                            atomicAdd(&hb[f], (unsigned long long)ov);
+                        } else {
+                            atomicAdd(&a.counts[rowcell + f], 1ull);
+                            atomicAdd(&a.bp[rowcell + f], (unsigned long long)ov);
+                        }
+                    }
+                }
+                if (more != ~0ull) break;  // a lane reached q.e or the segment end
+                p += 64;
+            }
+        }
+    }
+    if (LH) {
+        __syncthreads();
+        for (uint32_t f = tid; f < a.F; f += NT) {
+            const unsigned long long cv = hc[f], bv = hb[f];
+            if (cv | bv) {
+                atomicAdd(&a.counts[rowcell + f], cv);
+                atomicAdd(&a.bp[rowcell + f], bv);
+            }
+        }
+    }
+}
+
+constexpr uint32_t SPARSE_LDS_MAX_F = 2048;  // LDS histogram of all files up to this many
+
+template <int ITEMS>
+void launch_sweep(const SweepArgs& args, uint32_t grid, hipStream_t st) {
+    hipLaunchKernelGGL(sweep_kernel<ITEMS>, dim3(grid), dim3(NT), 0, st, args);
+}
+
+template <int ITEMS>
+int sweep_occupancy() {
+    int nb = 0;
+    CHK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, sweep_kernel<ITEMS>, NT, 0));
+    return std::max(1, nb);
 }
 
 }  // namespace
 
-void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp) {
+void query_plan(Query& q) {
     Index& ix = *q.idx;
-    hipStream_t st = ix.stream;
-    const uint32_t F = ix.nfiles, S = ix.nshards, nseg = q.nfiles * S;
-    const size_t out_bytes = (size_t)q.nfiles * F * sizeof(uint64_t);
-    CHK_HIP(hipMemsetAsync(d_counts, 0, out_bytes, st));
-    CHK_HIP(hipMemsetAsync(d_bp, 0, out_bytes, st));
-    // Host job list from the segment table read back by query_prepare.
+    if (q.plan_version == ix.tune_version) return;
+    const uint32_t S = ix.nshards, NG = ix.ngroups;
+    const Tuning& tu = ix.tune;
+    q.items = tu.items ? tu.items : 8;
+    q.tile = NT * q.items;
+    // dense jobs: (dense file, shard, group, stream) with events and DB elements
     auto& J = q.jobs_host;
     J.clear();
-    uint32_t tiles = 0, bounds = 0;
-    for (uint32_t r = 0; r < q.nfiles; r++)
+    uint64_t tiles = 0, bounds = 0, ev = 0;
+    const uint32_t ND = (uint32_t)q.dense_rows.size();
+    for (uint32_t k = 0; k < ND; k++)
         for (uint32_t s = 0; s < S; s++) {
-            uint32_t a = q.seg_host[(size_t)r * S + s], z = q.seg_host[nseg + (size_t)r * S + s];
-            uint64_t nd = ix.shard_off[s + 1] - ix.shard_off[s];
-            if (z <= a || nd == 0) continue;  // shard absent from the query or the DB
-            CHK_REQUIRE(nd + (z - a) < (1ull << 32), CHK_ERR_INVALID, "shard too large for one job");
-            for (uint32_t e = 0; e < 2; e++) {
-                Job j{};
-                j.db_off = ix.shard_off[s];
-                j.n_db = (uint32_t)nd;
-                j.q_off = a;
-                j.m = z - a;
-                j.qfile = r;
-                j.is_end = e;
-                j.tile0 = tiles;
-                j.part0 = bounds;
-                uint32_t nt = (uint32_t)((nd + j.m + TILE - 1) / TILE);
-                tiles += nt;
-                bounds += nt + 1;
-                J.push_back(j);
+            const uint32_t m = 2 * q.seg_n[(size_t)k * S + s];
+            const uint64_t q_off = ev;
+            ev += m;
+            if (!m) continue;
+            for (uint32_t g = 0; g < NG; g++) {
+                const uint64_t db0 = ix.str_off[(size_t)s * NG + g];
+                const uint64_t nd = ix.str_off[(size_t)s * NG + g + 1] - db0;
+                if (!nd) continue;
+                CHK_REQUIRE(nd + m < (1ull << 32), CHK_ERR_INVALID, "shard too large for one job");
+                for (uint32_t e = 0; e < 2; e++) {
+                    Job j{};
+                    j.db_off = db0;
+                    j.n_db = (uint32_t)nd;
+                    j.q_off = (uint32_t)q_off;
+                    j.m = m;
+                    j.row = q.dense_rows[k];
+                    j.group = g;
+                    j.is_end = e;
+                    j.tile0 = (uint32_t)tiles;
+                    j.part0 = (uint32_t)bounds;
+                    const uint64_t nt = (nd + m + q.tile - 1) / q.tile;
+                    tiles += nt;
+                    bounds += nt + 1;
+                    J.push_back(j);
+                }
             }
         }
-    if (J.empty()) return;
-    q.jobs.reserve(J.size());
-    q.part.reserve(bounds);
-    CHK_HIP(hipMemcpyAsync(q.jobs.p, J.data(), J.size() * sizeof(Job), hipMemcpyHostToDevice, st));
-    hipEvent_t t0 = ix.prof.begin(st);
-    hipLaunchKernelGGL(partition_kernel, dim3((bounds + NT - 1) / NT), dim3(NT), 0, st, q.jobs.p,
-                       (uint32_t)J.size(), bounds, ix.s_coord.p, ix.e_coord.p, q.key32.p, q.part.p);
-    CHK_HIP(hipGetLastError());
-    ix.prof.end("partition", t0, st);
-    size_t lds = 16ull * F + QS * 16ull + MAXJ * sizeof(LJob) + PWIN * 4ull;
-    CHK_REQUIRE(lds <= 64 * 1024, CHK_ERR_INVALID,
-                "too many DB files for the LDS histogram of one block (limit about 3000)");
-    uint32_t occ = std::max<uint32_t>(1, std::min<uint32_t>(8, (uint32_t)((160 * 1024) / (lds + 256))));
-    if (ix.tune.blocks_per_cu) occ = std::min(occ, ix.tune.blocks_per_cu);
-    uint32_t max_blocks = (uint32_t)ix.ncu * occ;
-    uint32_t tpb = std::max<uint32_t>(1, (tiles + max_blocks - 1) / max_blocks);
-    if (ix.tune.tiles_per_block) tpb = ix.tune.tiles_per_block;
-    const uint32_t pwin = ix.tune.part_window ? std::max<uint32_t>(2, std::min(PWIN, ix.tune.part_window)) : PWIN;
-    // a block may span at most MAXJ jobs
-    auto max_span = [&](uint32_t tp) {
-        uint32_t worst = 0, j = 0;
-        for (uint32_t t0 = 0; t0 < tiles; t0 += tp) {
-            uint32_t t1 = std::min(tiles, t0 + tp) - 1;
-            while (j + 1 < J.size() && J[j + 1].tile0 <= t0) j++;
-            uint32_t j2 = j;
-            while (j2 + 1 < J.size() && J[j2 + 1].tile0 <= t1) j2++;
-            worst = std::max(worst, j2 - j + 1);
-        }
-        return worst;
-    };
-    while (tpb > 1 && max_span(tpb) > MAXJ) tpb = (tpb + 1) / 2;
-    uint32_t nblocks = (tiles + tpb - 1) / tpb;
-    SweepArgs args{q.jobs.p, (uint32_t)J.size(), tiles, tpb, F, pwin, q.part.p, ix.s_coord.p,
-                   ix.e_coord.p, nullptr, nullptr, q.key32.p, q.rs.p, q.w.p,
-                   reinterpret_cast<unsigned long long*>(d_counts),
-                   reinterpret_cast<unsigned long long*>(d_bp)};
-    t0 = ix.prof.begin(st);
-    if (ix.fid16) {
-        args.s_fid = ix.s_fid16.p;
-        args.e_fid = ix.e_fid16.p;
-        hipLaunchKernelGGL(sweep_kernel<uint16_t>, dim3(nblocks), dim3(NT), lds, st, args);
-    } else {
-        args.s_fid = ix.s_fid.p;
-        args.e_fid = ix.e_fid.p;
-        hipLaunchKernelGGL(sweep_kernel<uint32_t>, dim3(nblocks), dim3(NT), lds, st, args);
+    CHK_REQUIRE(bounds < (1ull << 32), CHK_ERR_INVALID, "query batch too large for one sweep");
+    q.ntiles = (uint32_t)tiles;
+    q.nbounds = (uint32_t)bounds;
+    int occ = 1;
+    switch (q.items) {
+        case 4: occ = sweep_occupancy<4>(); break;
+        case 8: occ = sweep_occupancy<8>(); break;
+        case 16: occ = sweep_occupancy<16>(); break;
+        default: CHK_REQUIRE(false, CHK_ERR_INVALID, "sweep.items must be 4, 8 or 16");
     }
-    CHK_HIP(hipGetLastError());
-    ix.prof.end("sweep", t0, st);
+    if (tu.blocks_per_cu) occ = std::min<int>(occ, (int)tu.blocks_per_cu);
+    const uint64_t max_grid = (uint64_t)ix.ncu * occ;
+    q.chunk = tu.chunk ? std::min<uint32_t>(tu.chunk, MAXC)
+                       : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(MAXC, tiles / (max_grid * 4)));
+    q.nchunks = (uint32_t)((tiles + q.chunk - 1) / q.chunk);
+    q.grid = (uint32_t)std::min<uint64_t>(max_grid, q.nchunks);
+    auto& CJ = q.chunk_job_host;
+    CJ.assign(q.nchunks + 1, 0);
+    {
+        uint32_t j = 0;
+        for (uint32_t c = 0; c < q.nchunks; c++) {
+            const uint64_t t = (uint64_t)c * q.chunk;
+            while (j + 1 < J.size() && J[j + 1].tile0 <= t) j++;
+            CJ[c] = j;
+        }
+        CJ[q.nchunks] = J.empty() ? 0 : (uint32_t)J.size() - 1;
+    }
+    // sparse items
+    const uint32_t spi = tu.sparse_items ? tu.sparse_items : SPI;
+    auto& I = q.items_host;
+    I.clear();
+    {
+        uint64_t p = q.n_dense;
+        for (uint32_t r = 0; r < q.nfiles; r++) {
+            if (q.file_dense[r] || !q.file_n[r]) continue;
+            for (uint64_t b = 0; b < q.file_n[r]; b += spi) {
+                SparseItem it{};
+                it.row = r;
+                it.begin = (uint32_t)(p + b);
+                it.end = (uint32_t)(p + std::min<uint64_t>(q.file_n[r], b + spi));
+                I.push_back(it);
+            }
+            p += q.file_n[r];
+        }
+    }
+    hipStream_t st = ix.stream;
+    q.jobs.reserve(J.size());
+    q.chunk_job.reserve(CJ.size());
+    q.part.reserve(std::max<uint32_t>(1, q.nbounds));
+    q.queue.reserve(2);
+    q.sitems.reserve(I.size());
+    if (!J.empty()) {
+        CHK_HIP(hipMemcpyAsync(q.jobs.p, J.data(), J.size() * sizeof(Job), hipMemcpyHostToDevice, st));
+        CHK_HIP(hipMemcpyAsync(q.chunk_job.p, CJ.data(), CJ.size() * 4, hipMemcpyHostToDevice, st));
+    }
+    if (!I.empty())
+        CHK_HIP(hipMemcpyAsync(q.sitems.p, I.data(), I.size() * sizeof(SparseItem),
+                               hipMemcpyHostToDevice, st));
+    CHK_HIP(hipStreamSynchronize(st));
+    q.plan_version = ix.tune_version;
+}
+
+void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
+    Index& ix = *q.idx;
+    CHK_REQUIRE(q.plan_version == ix.tune_version, CHK_ERR_INTERNAL, "sweep plan is stale");
+    const uint32_t F = ix.nfiles;
+    const size_t cells = (size_t)q.nfiles * F;
+    zero_u64(d_counts, cells, st);
+    zero_u64(d_bp, cells, st);
+    if (!q.jobs_host.empty()) {
+        zero_u64(reinterpret_cast<uint64_t*>(q.queue.p), 1, st);
+        hipEvent_t t0 = ix.prof.begin(st);
+        query_events(q, st);
+        ix.prof.end("qprep", t0, st);
+        t0 = ix.prof.begin(st);
+        hipLaunchKernelGGL(partition_kernel, dim3((q.nbounds + NT - 1) / NT), dim3(NT), 0, st,
+                           q.jobs.p, (uint32_t)q.jobs_host.size(), q.nbounds, q.tile, ix.s_coord.p,
+                           ix.e_coord.p, q.key32.p, q.part.p);
+        CHK_HIP(hipGetLastError());
+        ix.prof.end("partition", t0, st);
+        SweepArgs args{q.jobs.p, q.chunk_job.p, q.part.p, q.ntiles, q.nchunks, q.chunk, F,
+                       ix.s_coord.p, ix.e_coord.p, ix.s_lfid.p, ix.e_lfid.p, q.key32.p, q.rs.p,
+                       q.w.p, q.queue.p,
+                       reinterpret_cast<unsigned long long*>(d_counts),
+                       reinterpret_cast<unsigned long long*>(d_bp)};
+        t0 = ix.prof.begin(st);
+        switch (q.items) {
+            case 4: launch_sweep<4>(args, q.grid, st); break;
+            case 8: launch_sweep<8>(args, q.grid, st); break;
+            default: launch_sweep<16>(args, q.grid, st); break;
+        }
+        CHK_HIP(hipGetLastError());
+        ix.prof.end("sweep", t0, st);
+    }
+    if (!q.items_host.empty()) {
+        SparseArgs sa{q.sitems.p, q.shard.p, q.start.p, q.end.p, ix.d_lseg.p, ix.nlayers, F,
+                      ix.l_start.p, ix.l_end.p, ix.l_fid.p, ix.jt.p,
+                      reinterpret_cast<unsigned long long*>(d_counts),
+                      reinterpret_cast<unsigned long long*>(d_bp)};
+        const uint32_t ni = (uint32_t)q.items_host.size();
+        hipEvent_t t0 = ix.prof.begin(st);
+        if (F <= SPARSE_LDS_MAX_F)
+            hipLaunchKernelGGL(sparse_kernel<true>, dim3(ni), dim3(NT), 16 * F, st, sa);
+        else
+            hipLaunchKernelGGL(sparse_kernel<false>, dim3(ni), dim3(NT), 0, st, sa);
+        CHK_HIP(hipGetLastError());
+        ix.prof.end("sparse", t0, st);
+    }
 }
 
 }  // namespace chk

@@ -13,9 +13,17 @@ pytestmark = pytest.mark.gpu
 
 
 def gpu_sweep(chk, db, q, nshards, F, nq):
+    """Sweep with the dense merge-path path and with the sparse pair path;
+    both must agree exactly before either is compared with the oracle."""
     ix = chk.Index(db[0], db[1], db[2], db[3], nshards, F)
-    qq = ix.query(q[0], q[1], q[2], q[3] if nq > 1 else None, nq)
-    return ix.sweep(qq)
+    out = []
+    for method in (1, 2):
+        ix.tune("sweep.method", method)
+        qq = ix.query(q[0], q[1], q[2], q[3] if nq > 1 else None, nq)
+        out.append(ix.sweep(qq))
+    (c1, b1), (c2, b2) = out
+    assert (c1 == c2).all() and (b1 == b2).all(), "dense and sparse paths disagree"
+    return c1, b1
 
 
 def one(shard_start_end_sid):
@@ -154,20 +162,82 @@ def test_repeat_is_deterministic(chk, orc):
     assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
 
 
-@pytest.mark.parametrize("tpb,pwin,bpc", [(3, 4, 1), (40, 2, 2), (1, 2, 0)])
-def test_launch_shapes_agree(chk, orc, tpb, pwin, bpc):
-    # tiny partition windows and tile ranges force the LDS window refills,
-    # job-span changes and query-file switches inside one block
+@pytest.mark.parametrize("chunk,items,bpc", [(1, 4, 1), (3, 8, 2), (64, 16, 0), (2, 8, 0)])
+def test_launch_shapes_agree(chk, orc, chunk, items, bpc):
+    # small chunks and tiles force work-queue items that span several jobs,
+    # query-file switches inside one block and the pipeline's odd tail
     rng = np.random.default_rng(21)
     F, nq, S = 12, 5, 7
     db = flatten(random_files(rng, F, S, (2000, 9000), 3_000_000, 8_000))
     q = flatten(random_files(rng, nq, S, (100, 20_000), 3_000_000, 8_000))
     ix = chk.Index(*db, S, F)
-    ix.tune("sweep.tiles_per_block", tpb)
-    ix.tune("sweep.part_window", pwin)
+    ix.tune("sweep.method", 1)
+    ix.tune("sweep.chunk", chunk)
+    ix.tune("sweep.items", items)
     ix.tune("sweep.blocks_per_cu", bpc)
     c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3], nq))
     oc, obp = orc.sweep_ranks(db, q, F, nq, S)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+@pytest.mark.parametrize("spi", [1, 7, 256])
+def test_sparse_item_sizes_agree(chk, orc, spi):
+    rng = np.random.default_rng(22)
+    F, nq, S = 9, 4, 3
+    db = flatten(random_files(rng, F, S, (1000, 5000), 1_000_000, 50_000))
+    q = flatten(random_files(rng, nq, S, (1, 900), 1_000_000, 50_000))
+    ix = chk.Index(*db, S, F)
+    ix.tune("sweep.method", 2)
+    ix.tune("sweep.sparse_items", spi)
+    c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3], nq))
+    oc, obp = orc.sweep_ranks(db, q, F, nq, S)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+def test_mixed_batch_chooses_per_file(chk, orc):
+    # file sizes straddle the sparse/dense threshold: both kernels write
+    # rows of the same output in one sweep
+    rng = np.random.default_rng(23)
+    F, S = 30, 4
+    db = flatten(random_files(rng, F, S, (500, 4000), 4_000_000, 20_000))
+    sizes = [3, 40, 400, 4000, 20_000, 1, 90_000]
+    q = flatten([random_files(rng, 1, S, n, 4_000_000, 20_000)[0] for n in sizes])
+    ix = chk.Index(*db, S, F)
+    ix.tune("sweep.sparse_max", 500)
+    c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3], len(sizes)))
+    oc, obp = orc.sweep_ranks(db, q, F, len(sizes), S)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+@pytest.mark.parametrize("F", [1500, 2600])
+def test_many_files_use_file_groups(chk, orc, F):
+    # more DB files than one LDS histogram holds: dense streams split into
+    # groups of 1024 files; the sparse path switches to direct atomics above
+    # 2048 files
+    rng = np.random.default_rng(24 + F)
+    S = 2
+    db = flatten(random_files(rng, F, S, (5, 60), 2_000_000, 30_000))
+    q = flatten(random_files(rng, 3, S, (2000, 6000), 2_000_000, 30_000))
+    c, bp = gpu_sweep(chk, db, q, S, F, 3)
+    oc, obp = orc.sweep_ranks(db, q, F, 3, S)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+def test_lengths_across_all_layers(chk, orc):
+    # DB lengths from 1 bp to 5 Mb populate many size layers; the sparse
+    # path's look-back per layer must still find every overlap
+    rng = np.random.default_rng(25)
+    F, S, span = 6, 2, 50_000_000
+    files = []
+    for _ in range(F):
+        n = 3000
+        ln = np.exp(rng.uniform(0, np.log(5e6), n)).astype(np.uint32) + 1
+        s = rng.integers(0, span, n).astype(np.uint32)
+        files.append((rng.integers(0, S, n).astype(np.uint32), s, s + ln))
+    db = flatten(files)
+    q = flatten(random_files(rng, 2, S, 2000, span, 200_000))
+    c, bp = gpu_sweep(chk, db, q, S, F, 2)
+    oc, obp = orc.sweep_ranks(db, q, F, 2, S)
     assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
 
 
