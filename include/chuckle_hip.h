@@ -175,6 +175,36 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
  * others to every later sweep.
  */
 int chk_tune(chk_index* idx, const char* key, int64_t value);
+/* Current value of a tuning key (0 = automatic choice). */
+int chk_tune_get(chk_index* idx, const char* key, int64_t* value);
+
+/*
+ * Autotuner (no counterpart in the reference, whose sweep has no launch
+ * shape): coordinate descent over sweep.sparse_max, sweep.items, sweep.chunk
+ * and sweep.blocks_per_cu on this query batch.  Each candidate is swept once
+ * to warm up, then `reps` times, timed with HIP events; the fastest value of
+ * each knob is kept in the index's tuning (read it back with chk_tune_get).
+ * d_counts / d_bp are device outputs of the batch's size (overwritten).
+ * best_ms receives the best time per sweep; report (optional, report_len
+ * bytes) one line "knob=value: ms" per candidate.
+ */
+int chk_autotune(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_bp, int reps,
+                 double* best_ms, char* report, uint64_t report_len);
+
+/*
+ * Batched form as one HIP graph: the whole sweep of a query batch (output
+ * zeroing, event sort, prefix sums, merge-path partition, the persistent
+ * work-queue dense sweep and the sparse pair sweep) and, when d_p is given,
+ * the p-value kernel, captured on the index's stream.  Replays need no host
+ * work and no allocation.  The graph reads the query's device buffers, so
+ * it sees new contents uploaded into them; it must be recreated after the
+ * index's tuning changes (chk_graph_launch then reports an error).
+ */
+typedef struct chk_graph chk_graph;
+int chk_graph_create(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_bp,
+                     double* d_p, double* d_llr, uint8_t* d_flags, chk_graph** out);
+int chk_graph_launch(chk_graph* g);
+int chk_graph_free(chk_graph* g);
 
 /* Launch a plain device copy kernel ("calib_copy") that reads and writes
  * `bytes` bytes once; used to scale hardware byte counters. */

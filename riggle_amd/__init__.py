@@ -69,6 +69,11 @@ def _declare(L):
     L.chk_moments_build.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.chk_moments_download.argtypes = [vp, C.POINTER(u64), vp, vp, vp, vp]
     L.chk_query_stats.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.chk_tune_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+    L.chk_autotune.argtypes = [vp, vp, vp, vp, i32, C.POINTER(C.c_double), C.c_char_p, u64]
+    L.chk_graph_create.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp)]
+    L.chk_graph_launch.argtypes = [vp]
+    L.chk_graph_free.argtypes = [vp]
 
 
 def _check(code):
@@ -196,6 +201,25 @@ class Index:
     def tune(self, key, value):
         _check(lib().chk_tune(self._h, key.encode(), int(value)))
 
+    def tune_get(self, key):
+        v = C.c_int64()
+        _check(lib().chk_tune_get(self._h, key.encode(), C.byref(v)))
+        return v.value
+
+    def autotune(self, query, d_counts, d_bp, reps=3):
+        """Pick the sweep knobs for this batch by timing; returns
+        (best ms per sweep, chosen knob values, report lines)."""
+        best = C.c_double()
+        buf = C.create_string_buffer(16384)
+        _check(lib().chk_autotune(self._h, query.handle, d_counts.ptr, d_bp.ptr, int(reps),
+                                  C.byref(best), buf, len(buf)))
+        keys = ("sweep.sparse_max", "sweep.items", "sweep.chunk", "sweep.blocks_per_cu")
+        return best.value, {k: self.tune_get(k) for k in keys}, buf.value.decode().splitlines()
+
+    def graph(self, query, d_counts, d_bp, d_p=None, d_llr=None, d_flags=None):
+        """Capture the batch's sweep (and p-values) as one replayable HIP graph."""
+        return Graph(self, query, d_counts, d_bp, d_p, d_llr, d_flags)
+
     def calibrate_copy(self, nbytes):
         _check(lib().chk_calibrate_copy(self._h, nbytes))
 
@@ -235,6 +259,32 @@ class Query:
     def close(self):
         if getattr(self, "_h", None):
             lib().chk_query_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """One captured sweep (+ p-values) of a query batch; launch() replays it on
+    the index's stream."""
+
+    def __init__(self, index, query, d_counts, d_bp, d_p=None, d_llr=None, d_flags=None):
+        h = C.c_void_p()
+        ptr = lambda b: b.ptr if b is not None else None
+        _check(lib().chk_graph_create(index.handle, query.handle, d_counts.ptr, d_bp.ptr,
+                                      ptr(d_p), ptr(d_llr), ptr(d_flags), C.byref(h)))
+        self._h, self._index, self._query = h, index, query
+
+    def launch(self):
+        _check(lib().chk_graph_launch(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().chk_graph_free(self._h)
             self._h = None
 
     def __del__(self):

@@ -12,6 +12,17 @@ struct chk_index {
 struct chk_query {
     chk::Query q;
 };
+struct chk_graph {
+    chk_index* idx = nullptr;
+    chk_query* q = nullptr;
+    uint32_t plan_version = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~chk_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
 
 namespace chk {
 static thread_local std::string g_last_error;
@@ -227,7 +238,7 @@ int chk_pvalues_async(chk_index* idx, chk_query* q, const uint64_t* d_counts, co
                     "NULL argument");
         CHK_REQUIRE(q->q.idx == &idx->ix, CHK_ERR_INVALID, "query belongs to another index");
         set_device(idx->ix.device);
-        chk::pvalues_run(q->q, d_counts, d_bp, d_p, d_llr, d_flags);
+        chk::pvalues_run(q->q, d_counts, d_bp, d_p, d_llr, d_flags, idx->ix.stream);
     });
 }
 
@@ -249,7 +260,7 @@ int chk_query_stats(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp
         df.reserve(cells);
         chk::query_plan(q->q);
         chk::sweep_run(q->q, dc.p, db.p, idx->ix.stream);
-        chk::pvalues_run(q->q, dc.p, db.p, dp.p, dl.p, df.p);
+        chk::pvalues_run(q->q, dc.p, db.p, dp.p, dl.p, df.p, st);
         CHK_HIP(hipMemcpyAsync(counts, dc.p, cells * 8, hipMemcpyDeviceToHost, st));
         CHK_HIP(hipMemcpyAsync(bp, db.p, cells * 8, hipMemcpyDeviceToHost, st));
         CHK_HIP(hipMemcpyAsync(p, dp.p, cells * 8, hipMemcpyDeviceToHost, st));
@@ -350,6 +361,103 @@ int chk_tune(chk_index* idx, const char* key, int64_t value) {
         } else if (k == "sweep.sparse_items") t.sparse_items = v;
         else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
         idx->ix.tune_version++;
+    });
+}
+
+int chk_tune_get(chk_index* idx, const char* key, int64_t* value) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && key && value, CHK_ERR_INVALID, "NULL argument");
+        std::string k(key);
+        const chk::Tuning& t = idx->ix.tune;
+        if (k == "sweep.blocks_per_cu") *value = t.blocks_per_cu;
+        else if (k == "sweep.chunk") *value = t.chunk;
+        else if (k == "sweep.items") *value = t.items;
+        else if (k == "sweep.sparse_max") *value = (int64_t)t.sparse_max;
+        else if (k == "sweep.method") *value = t.method;
+        else if (k == "sweep.sparse_items") *value = t.sparse_items;
+        else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
+    });
+}
+
+int chk_autotune(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_bp, int reps,
+                 double* best_ms, char* report, uint64_t report_len) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && q && d_counts && d_bp, CHK_ERR_INVALID, "NULL argument");
+        CHK_REQUIRE(q->q.idx == &idx->ix, CHK_ERR_INVALID, "query belongs to another index");
+        set_device(idx->ix.device);
+        std::string rep;
+        const double ms = chk::autotune(q->q, d_counts, d_bp, reps, &rep);
+        if (best_ms) *best_ms = ms;
+        if (report && report_len) {
+            const size_t n = std::min<size_t>(rep.size(), (size_t)report_len - 1);
+            std::memcpy(report, rep.data(), n);
+            report[n] = 0;
+        }
+    });
+}
+
+int chk_graph_create(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_bp,
+                     double* d_p, double* d_llr, uint8_t* d_flags, chk_graph** out) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && q && d_counts && d_bp && out, CHK_ERR_INVALID, "NULL argument");
+        CHK_REQUIRE(q->q.idx == &idx->ix, CHK_ERR_INVALID, "query belongs to another index");
+        CHK_REQUIRE((d_p && d_llr && d_flags) || (!d_p && !d_llr && !d_flags), CHK_ERR_INVALID,
+                    "p-value outputs must be all given or all NULL");
+        *out = nullptr;
+        set_device(idx->ix.device);
+        chk::Index& ix = idx->ix;
+        hipStream_t st = ix.stream;
+        // one eager pass sizes every work buffer, so the capture allocates nothing
+        chk::query_plan(q->q);
+        chk::sweep_run(q->q, d_counts, d_bp, st);
+        if (d_p) chk::pvalues_run(q->q, d_counts, d_bp, d_p, d_llr, d_flags, st);
+        CHK_HIP(hipStreamSynchronize(st));
+        auto* g = new chk_graph();
+        const bool prof = ix.prof.on;
+        ix.prof.on = false;
+        try {
+            g->idx = idx;
+            g->q = q;
+            g->plan_version = q->q.plan_version;
+            CHK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                chk::sweep_run(q->q, d_counts, d_bp, st);
+                if (d_p) chk::pvalues_run(q->q, d_counts, d_bp, d_p, d_llr, d_flags, st);
+            } catch (...) {
+                hipGraph_t dead = nullptr;
+                (void)hipStreamEndCapture(st, &dead);
+                if (dead) (void)hipGraphDestroy(dead);
+                throw;
+            }
+            CHK_HIP(hipStreamEndCapture(st, &g->graph));
+            CHK_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        } catch (...) {
+            ix.prof.on = prof;
+            delete g;
+            throw;
+        }
+        ix.prof.on = prof;
+        *out = g;
+    });
+}
+
+int chk_graph_launch(chk_graph* g) {
+    return guarded([&] {
+        CHK_REQUIRE(g && g->exec, CHK_ERR_INVALID, "graph is NULL");
+        CHK_REQUIRE(g->q->q.plan_version == g->idx->ix.tune_version &&
+                        g->plan_version == g->idx->ix.tune_version,
+                    CHK_ERR_INVALID, "the index's tuning changed after capture: recreate the graph");
+        set_device(g->idx->ix.device);
+        CHK_HIP(hipGraphLaunch(g->exec, g->idx->ix.stream));
+    });
+}
+
+int chk_graph_free(chk_graph* g) {
+    return guarded([&] {
+        if (!g) return;
+        set_device(g->idx->ix.device);
+        (void)hipStreamSynchronize(g->idx->ix.stream);
+        delete g;
     });
 }
 

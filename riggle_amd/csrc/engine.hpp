@@ -154,6 +154,8 @@ struct Query {
     Index* idx = nullptr;
     uint32_t nfiles = 0;
     uint64_t n = 0;                   // intervals kept (on shards of the DB)
+    std::vector<uint32_t> h_shard, h_start, h_end, h_row;  // host copy, input order
+    bool placed = false;
     uint64_t n_dense = 0;             // the first n_dense belong to dense files
     std::vector<uint64_t> file_n;     // intervals per query file
     std::vector<uint8_t> file_dense;  // method per query file
@@ -201,13 +203,16 @@ void moments_build(Index& ix);
 void moments_download(Index& ix, int64_t* slot_off, uint32_t* slot_len, double* mean, double* var);
 // p-values for every cell with count > 0 (device arrays of nfiles(q) x F).
 void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, double* d_p,
-                 double* d_llr, uint8_t* d_flags);
+                 double* d_llr, uint8_t* d_flags, hipStream_t st);
 
 void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                  const uint32_t* fid);
 void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                   const uint32_t* qfile);
-// Host work plan of the dense and sparse paths for the current tuning.
+// Method per file and device placement for the current tuning.
+void query_place(Query& q);
+// Placement plus the host work plan of the dense and sparse paths for the
+// current tuning (no-op when the tuning is unchanged).
 void query_plan(Query& q);
 // One sweep: zero the outputs, sort the dense files' events, prefix sums,
 // merge-path partition, dense sweep, sparse sweep.  Enqueued on `st` with
@@ -215,6 +220,11 @@ void query_plan(Query& q);
 void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st);
 // Fill n u64 with zero from a kernel (graph-safe clearing).
 void zero_u64(uint64_t* p, size_t n, hipStream_t st);
+// Mean time of `reps` sweeps (after one warm-up) with HIP events.
+double time_sweeps(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps);
+// Coordinate-descent autotuner over the sweep knobs for this batch; leaves the
+// best values in the index's tuning and returns the best time per sweep.
+double autotune(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps, std::string* report);
 // Plain copy of `bytes` (reads and writes that many bytes once).
 void calibrate_copy(Index& ix, uint64_t bytes);
 

@@ -252,73 +252,84 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
     ix.scratch.vals_alt.release();
 }
 
-// Upload one batch of query files.  Intervals on shards absent from the DB
-// are dropped (the reference skips such shards, src/tasks/query/mod.rs:193-199);
-// the rest are grouped by file, dense files first, so each sparse file is a
-// contiguous range and the dense files' events are a prefix.
+// Take one batch of query files.  Intervals on shards absent from the DB are
+// dropped (the reference skips such shards, src/tasks/query/mod.rs:193-199);
+// the rest are kept on the host and placed on the device by query_place.
 void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                   const uint32_t* qfile) {
     Index& ix = *q.idx;
     const uint64_t n_in = q.n;
     const uint32_t S = ix.nshards, R = q.nfiles;
     q.file_n.assign(R, 0);
+    q.h_shard.clear();
+    q.h_start.clear();
+    q.h_end.clear();
+    q.h_row.clear();
     for (uint64_t i = 0; i < n_in; i++) {
         CHK_REQUIRE(start[i] < end[i], CHK_ERR_INVALID, "query interval start must be less than end");
         CHK_REQUIRE(end[i] <= 0x7FFFFFFFu, CHK_ERR_INVALID,
                     "query coordinate exceeds 2^31-1 (device coordinates are int32)");
         const uint32_t r = qfile ? qfile[i] : 0u;
         CHK_REQUIRE(r < R, CHK_ERR_INVALID, "query file id out of range");
-        if (shard[i] < S && ix.shard_n[shard[i]]) q.file_n[r]++;
+        if (shard[i] >= S || !ix.shard_n[shard[i]]) continue;
+        q.file_n[r]++;
+        q.h_shard.push_back(shard[i]);
+        q.h_start.push_back(start[i]);
+        q.h_end.push_back(end[i]);
+        q.h_row.push_back(r);
     }
-    // method per file: the sparse path enumerates pairs through the layered
-    // table, the dense path streams the whole DB; a file goes sparse when it
-    // is small against the DB
+    q.n = q.h_start.size();
+    CHK_REQUIRE(q.n < (1ull << 32), CHK_ERR_INVALID, "query batch exceeds 2^32 intervals");
+    q.placed = false;
+    q.plan_version = 0xFFFFFFFFu;
+    query_plan(q);
+}
+
+// Method per file and device placement: the sparse path enumerates pairs
+// through the layered table, the dense path streams the whole DB, so a file
+// goes sparse when it is small against the DB.  Intervals are grouped by
+// file, dense files first, so each sparse file is a contiguous range and the
+// dense files' events are a prefix.  Redone only when the choice changes.
+void query_place(Query& q) {
+    Index& ix = *q.idx;
+    const uint32_t S = ix.nshards, R = q.nfiles;
     const uint64_t smax = ix.tune.sparse_max ? ix.tune.sparse_max : std::max<uint64_t>(1, ix.n / 1024);
-    q.file_dense.assign(R, 0);
+    std::vector<uint8_t> dense(R, 0);
+    for (uint32_t r = 0; r < R; r++)
+        dense[r] = q.file_n[r] &&
+                   (ix.tune.method == 1 || (ix.tune.method == 0 && q.file_n[r] > smax));
+    if (q.placed && dense == q.file_dense) return;
+    q.file_dense = dense;
     q.dense_rows.clear();
     std::vector<uint32_t> drank(R, 0xFFFFFFFFu);
-    for (uint32_t r = 0; r < R; r++) {
-        bool dense = ix.tune.method == 1 || (ix.tune.method == 0 && q.file_n[r] > smax);
-        if (q.file_n[r] == 0) dense = false;
-        q.file_dense[r] = dense ? 1 : 0;
-        if (dense) {
+    for (uint32_t r = 0; r < R; r++)
+        if (dense[r]) {
             drank[r] = (uint32_t)q.dense_rows.size();
             q.dense_rows.push_back(r);
         }
-    }
     const uint32_t ND = (uint32_t)q.dense_rows.size();
-    // placement: dense files in file order, then sparse files in file order
-    std::vector<uint64_t> pos(R + 1, 0);
-    {
-        uint64_t p = 0;
-        for (uint32_t r = 0; r < R; r++)
-            if (q.file_dense[r]) { pos[r] = p; p += q.file_n[r]; }
-        q.n_dense = p;
-        for (uint32_t r = 0; r < R; r++)
-            if (!q.file_dense[r]) { pos[r] = p; p += q.file_n[r]; }
-        q.n = p;
-    }
+    std::vector<uint64_t> pos(R, 0);
+    uint64_t p = 0;
+    for (uint32_t r = 0; r < R; r++)
+        if (dense[r]) { pos[r] = p; p += q.file_n[r]; }
+    q.n_dense = p;
+    for (uint32_t r = 0; r < R; r++)
+        if (!dense[r]) { pos[r] = p; p += q.file_n[r]; }
     CHK_REQUIRE(2 * q.n_dense < (1ull << 32) - 1, CHK_ERR_INVALID,
                 "query batch too large (2 x dense intervals must fit u32)");
-    CHK_REQUIRE(q.n < (1ull << 32), CHK_ERR_INVALID, "query batch exceeds 2^32 intervals");
     const uint64_t n = q.n;
     std::vector<uint32_t> hs(n), hb(n), he(n), hr(n), hg(q.n_dense);
     q.seg_n.assign((size_t)ND * S, 0);
-    {
-        std::vector<uint64_t> fill(pos.begin(), pos.begin() + R);
-        for (uint64_t i = 0; i < n_in; i++) {
-            const uint32_t s = shard[i];
-            if (s >= S || !ix.shard_n[s]) continue;
-            const uint32_t r = qfile ? qfile[i] : 0u;
-            const uint64_t k = fill[r]++;
-            hs[k] = s;
-            hb[k] = start[i];
-            he[k] = end[i];
-            hr[k] = r;
-            if (q.file_dense[r]) {
-                hg[k] = drank[r] * S + s;
-                q.seg_n[(size_t)drank[r] * S + s]++;
-            }
+    for (uint64_t i = 0; i < n; i++) {
+        const uint32_t r = q.h_row[i], s = q.h_shard[i];
+        const uint64_t k = pos[r]++;
+        hs[k] = s;
+        hb[k] = q.h_start[i];
+        he[k] = q.h_end[i];
+        hr[k] = r;
+        if (dense[r]) {
+            hg[k] = drank[r] * S + s;
+            q.seg_n[(size_t)drank[r] * S + s]++;
         }
     }
     hipStream_t st = ix.stream;
@@ -341,8 +352,7 @@ void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const 
     q.rs.reserve(m + 1);
     q.w.reserve(m + 1);
     CHK_HIP(hipStreamSynchronize(st));
-    q.plan_version = 0xFFFFFFFFu;
-    query_plan(q);
+    q.placed = true;
 }
 
 // Sort the dense files' events and build the prefix sums (enqueued on st).

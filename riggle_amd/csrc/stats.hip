@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void pvalue_kernel(StatsArgs a) {
 }  // namespace
 
 void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, double* d_p,
-                 double* d_llr, uint8_t* d_flags) {
+                 double* d_llr, uint8_t* d_flags, hipStream_t st) {
     Index& ix = *q.idx;
     CHK_REQUIRE(ix.mom.present, CHK_ERR_NOT_FOUND, "the index has no moment tables");
     CHK_REQUIRE(q.has_groups, CHK_ERR_INVALID,
@@ -278,10 +278,10 @@ void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, doubl
                 q.g_shard.p, q.g_ss.p, q.g_ss2.p, q.g_cnt.p, ix.mom.d_nbins.p, ix.mom.d_slot_off.p,
                 ix.mom.d_slot_len.p, ix.mom.d_has_entry.p, ix.mom.mean.p, ix.mom.var.p, d_p, d_llr,
                 d_flags};
-    hipEvent_t t0 = ix.prof.begin(ix.stream);
-    hipLaunchKernelGGL(pvalue_kernel, dim3((cells + 3) / 4), dim3(256), 0, ix.stream, a);
+    hipEvent_t t0 = ix.prof.begin(st);
+    hipLaunchKernelGGL(pvalue_kernel, dim3((cells + 3) / 4), dim3(256), 0, st, a);
     CHK_HIP(hipGetLastError());
-    ix.prof.end("pvalues", t0, ix.stream);
+    ix.prof.end("pvalues", t0, st);
 }
 
 }  // namespace chk

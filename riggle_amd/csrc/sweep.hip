@@ -42,7 +42,7 @@ namespace {
 constexpr int NT = 256;
 constexpr uint32_t QS = 512;    // events staged per tile on the fast path (nq + 1 <= QS)
 constexpr uint32_t MAXC = 64;   // tiles per work item at most
-constexpr uint32_t SPI = 256;   // query intervals per sparse work item (default)
+constexpr uint32_t SPI = 256;   // query intervals per sparse work item at most (default)
 
 __device__ inline uint32_t db_key(int32_t x) { return 2u * (uint32_t)x + 1u; }
 
@@ -399,10 +399,13 @@ struct SparseArgs {
     unsigned long long* bp;
 };
 
-// One block per work item; each wave takes query intervals of the item in
-// turn and, per layer, scans the DB starts in [q.s - lookback + 1, q.e)
-// 64 at a time.  With LH the pairs are reduced in an LDS histogram of all F
-// files (flushed once per item), else added to the output directly.
+// One block per work item.  The item's (query interval, size layer) pairs
+// are dealt to the block's waves; per pair a wave looks up the first DB start
+// that can reach the interval (start >= q.s - lookback + 1) in the layer's
+// jump table and scans the layer's starts up to q.e 64 at a time, loading the
+// next 64 starts while the current ones are tested.  With LH the pairs are
+// reduced in an LDS histogram of all F files (flushed once per item), else
+// added to the output directly.
 template <bool LH>
 __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
     extern __shared__ unsigned long long sh[];
@@ -412,43 +415,50 @@ __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
     const uint32_t wv = ufl(tid >> 6);
     const SparseItem it = a.items[blockIdx.x];
     const uint64_t rowcell = (uint64_t)it.row * a.F;
+    const uint32_t NL = a.nlayers;
     if (LH) {
         for (uint32_t f = tid; f < a.F; f += NT) hc[f] = hb[f] = 0ull;
         __syncthreads();
     }
-    for (uint32_t i = it.begin + wv; i < it.end; i += NT / 64) {
+    const uint32_t nunits = (it.end - it.begin) * NL;
+    for (uint32_t u = wv; u < nunits; u += NT / 64) {
+        const uint32_t qi = u / NL, K = u - qi * NL;
+        const uint32_t i = it.begin + qi;
         const uint32_t s = ufl(a.shard[i]);
+        const LayerSeg L = a.lseg[(size_t)s * NL + K];
+        if (L.n == 0) continue;
         const int32_t qs = (int32_t)ufl(a.start[i]), qe = (int32_t)ufl(a.end[i]);
-        for (uint32_t K = 0; K < a.nlayers; K++) {
-            const LayerSeg L = a.lseg[(size_t)s * a.nlayers + K];
-            if (L.n == 0) continue;
-            const int64_t lo64 = (int64_t)qs - (int64_t)L.lookback + 1;
-            const int32_t lo = lo64 > 0 ? (int32_t)lo64 : 0;
-            const uint32_t bkt = (uint32_t)lo >> L.shift;
-            uint32_t p = bkt < L.nb ? a.jt[L.jt + bkt] : L.n;
-            while (p < L.n) {
+        const int64_t lo64 = (int64_t)qs - (int64_t)L.lookback + 1;
+        const int32_t lo = lo64 > 0 ? (int32_t)lo64 : 0;
+        const uint32_t bkt = (uint32_t)lo >> L.shift;
+        uint32_t p = bkt < L.nb ? ufl(a.jt[L.jt + bkt]) : L.n;
+        if (p >= L.n) continue;
+        const int32_t* ls = a.l_start + L.off;
+        int32_t st = p + lane < L.n ? ls[p + lane] : 0x7FFFFFFF;
+        for (;;) {
+            const bool before_end = st < qe;  // lanes past the segment hold INT_MAX
+            const bool more = __ballot(before_end) == ~0ull;
+            const uint32_t pn = p + 64;
+            int32_t stn = 0x7FFFFFFF;
+            if (more && pn + lane < L.n) stn = ls[pn + lane];
+            if (before_end && st >= lo) {
                 const uint32_t idx = p + lane;
-                const bool valid = idx < L.n;
-                const int32_t st = valid ? a.l_start[L.off + idx] : 0x7FFFFFFF;
-                const bool before_end = valid && st < qe;
-                const uint64_t more = __ballot(before_end);
-                if (before_end && st >= lo) {
-                    const int32_t en = a.l_end[L.off + idx];
-                    if (en > qs) {
-                        const uint32_t f = a.l_fid[L.off + idx];
-                        const int64_t ov = (int64_t)min(qe, en) - (int64_t)max(qs, st);
-                        if (LH) {
-                            atomicAdd(&hc[f], 1ull);
-                            atomicAdd(&hb[f], (unsigned long long)ov);
-                        } else {
-                            atomicAdd(&a.counts[rowcell + f], 1ull);
-                            atomicAdd(&a.bp[rowcell + f], (unsigned long long)ov);
-                        }
+                const int32_t en = a.l_end[L.off + idx];
+                if (en > qs) {
+                    const uint32_t f = a.l_fid[L.off + idx];
+                    const int64_t ov = (int64_t)min(qe, en) - (int64_t)max(qs, st);
+                    if (LH) {
+                        atomicAdd(&hc[f], 1ull);
+                        atomicAdd(&hb[f], (unsigned long long)ov);
+                    } else {
+                        atomicAdd(&a.counts[rowcell + f], 1ull);
+                        atomicAdd(&a.bp[rowcell + f], (unsigned long long)ov);
                     }
                 }
-                if (more != ~0ull) break;  // a lane reached q.e or the segment end
-                p += 64;
             }
+            if (!more || pn >= L.n) break;
+            p = pn;
+            st = stn;
         }
     }
     if (LH) {
@@ -482,6 +492,7 @@ int sweep_occupancy() {
 void query_plan(Query& q) {
     Index& ix = *q.idx;
     if (q.plan_version == ix.tune_version) return;
+    query_place(q);
     const uint32_t S = ix.nshards, NG = ix.ngroups;
     const Tuning& tu = ix.tune;
     q.items = tu.items ? tu.items : 8;
@@ -547,8 +558,12 @@ void query_plan(Query& q) {
         }
         CJ[q.nchunks] = J.empty() ? 0 : (uint32_t)J.size() - 1;
     }
-    // sparse items
-    const uint32_t spi = tu.sparse_items ? tu.sparse_items : SPI;
+    // sparse items: enough of them to fill the device even for one small
+    // file, at most SPI intervals each
+    const uint64_t n_sparse = q.n - q.n_dense;
+    const uint32_t spi =
+        tu.sparse_items ? tu.sparse_items
+                        : (uint32_t)std::max<uint64_t>(4, std::min<uint64_t>(SPI, (n_sparse + ix.ncu * 16 - 1) / ((uint64_t)ix.ncu * 16)));
     auto& I = q.items_host;
     I.clear();
     {
