@@ -175,6 +175,72 @@ def batched_leg(ix, cfg, F, nshards, D):
             "batched_over_plain": ms_plain / ms_batched, "rows_match_single_launches": rows_ok}
 
 
+def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
+    """Serving form against the resident DB: query files of log-uniform 1k-5M
+    intervals (40 % hotspot) arrive as a Poisson stream at 65 % of the
+    measured capacity; latency is submission to completion."""
+    import threading
+    import riggle_amd
+    from riggle_amd import gen
+    t = time.time()
+    g = cfg["genome"]
+    hot = gen.hotspots(g, 0xC40C1E000 + 2)
+    sizes = gen.log_uniform_counts(pool, 1e3, 5e6, seed, stream=0xC5C5)
+    files = [gen.intervals(g, int(n), seed, 100 + i, 1000.0, 1.2, hotspot_frac=0.4, hot=hot)
+             for i, n in enumerate(sizes)]
+    log(f"serving: {pool} distinct query files ({int(sizes.sum())} intervals) in "
+        f"{time.time() - t:.1f}s")
+    srv = riggle_amd.Server(ix, slots=3, max_batch_files=16)
+    # capacity: closed loop over the pool, twice (the first pass warms up)
+    for rep in range(2):
+        t = time.perf_counter()
+        tk = [srv.submit(*f) for f in files]
+        for x in tk:
+            srv.wait(x)
+        dt = time.perf_counter() - t
+    cap = pool / dt
+    rate = 0.65 * cap
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    pick = rng.integers(0, pool, n_requests)
+    srv.reset_stats()
+    tickets = []
+    done = threading.Event()
+
+    def waiter():
+        k = 0
+        while k < n_requests:
+            if k < len(tickets):
+                srv.wait(tickets[k])
+                k += 1
+            else:
+                time.sleep(0.0002)
+        done.set()
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    due = t0
+    for i in range(n_requests):
+        due += gaps[i]
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            time.sleep(min(due - now, 0.001))
+        tickets.append(srv.submit(*files[pick[i]]))
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    swept = sum(len(files[i][1]) + D for i in pick)
+    return {"requests": n_requests, "distinct_files": pool, "sizes": "log-uniform 1k-5M intervals",
+            "capacity_req_per_s": cap, "arrival_rate_per_s": rate, "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "files_per_batch": st["files_per_batch"], "swept_per_s": st["swept_per_s"],
+            "offered_swept_per_s": swept / wall, "slots": 3, "max_batch_files": 16}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +252,8 @@ def main():
                     help="launch-shape knob key=value (see chk_tune), may repeat")
     ap.add_argument("--skip-stats", action="store_true", help="skip moments + p-value legs")
     ap.add_argument("--skip-batched", action="store_true", help="skip the batched 256-file leg")
+    ap.add_argument("--serve-requests", type=int, default=300,
+                    help="requests of the serving leg (0 skips it)")
     ap.add_argument("--calibrate-bytes", type=int, default=0,
                     help="also run the plain copy kernel over this many bytes (counter scaling)")
     args = ap.parse_args()
@@ -275,6 +343,9 @@ def main():
         rec["stats"] = stats_leg(ix, qq, cfg, F, args.steps)
     if not args.skip_batched and args.config == "c2":
         rec["batched"] = batched_leg(ix, cfg, F, nshards, D)
+    if args.serve_requests > 0 and args.config == "c2":
+        rec["serving"] = serving_leg(ix, cfg, F, D, args.serve_requests)
+        log(f"serving: {rec['serving']}")
 
     if not args.no_cpu:
         rec_cpu, cnt_cpu, sel = cpu_baseline((sh, s, e, fid), (qsh, qs, qe), nshards, F)

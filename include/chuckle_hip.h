@@ -206,6 +206,38 @@ int chk_graph_create(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t*
 int chk_graph_launch(chk_graph* g);
 int chk_graph_free(chk_graph* g);
 
+/*
+ * Serving form (no counterpart in the reference, which runs one `chuckle
+ * query` per process): a request loop that keeps the index resident, admits
+ * query files by HBM footprint and batches them as they arrive.  One
+ * scheduler thread owns the device work; `slots` batches (1..8) may be in
+ * flight at once, each on its own stream, so requests that arrive while
+ * others run are admitted as soon as a slot and budget free up and overlap
+ * on the device.  A batch takes at most max_batch_files waiting requests;
+ * hbm_budget bounds the device bytes of in-flight requests (0: half of the
+ * free memory at creation; a request above the budget runs alone).  With
+ * pvalues != 0 the index needs its genome and moment tables.  A request's
+ * results do not depend on the batch it shared.  While a server runs, do not
+ * enable profiling on its index.
+ */
+typedef struct chk_server chk_server;
+int chk_server_create(chk_index* idx, uint32_t slots, uint32_t max_batch_files,
+                      uint64_t hbm_budget, int pvalues, chk_server** out);
+/* Queue one query file (copied; returns at once) and get its ticket. */
+int chk_server_submit(chk_server* s, uint64_t n, const uint32_t* shard, const uint32_t* start,
+                      const uint32_t* end, uint64_t* ticket);
+/* Block until the request is done; outputs are rows of nfiles(index) entries
+ * (any may be NULL).  latency_ms = completion - submission. */
+int chk_server_wait(chk_server* s, uint64_t ticket, uint64_t* counts, uint64_t* bp, double* p,
+                    double* llr, uint8_t* flags, double* latency_ms);
+/* Latency percentiles (ms) over completed requests; swept_per_s =
+ * sum(|Q| + |DB|) over them / (last completion - first submission). */
+int chk_server_stats(chk_server* s, uint64_t* completed, double* p50_ms, double* p99_ms,
+                     double* mean_ms, double* swept_per_s, double* files_per_batch);
+int chk_server_reset_stats(chk_server* s);
+/* Waits for in-flight batches, then stops the loop. */
+int chk_server_free(chk_server* s);
+
 /* Launch a plain device copy kernel ("calib_copy") that reads and writes
  * `bytes` bytes once; used to scale hardware byte counters. */
 int chk_calibrate_copy(chk_index* idx, uint64_t bytes);

@@ -74,6 +74,13 @@ def _declare(L):
     L.chk_graph_create.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp)]
     L.chk_graph_launch.argtypes = [vp]
     L.chk_graph_free.argtypes = [vp]
+    dp_ = C.POINTER(C.c_double)
+    L.chk_server_create.argtypes = [vp, u32, u32, u64, i32, C.POINTER(vp)]
+    L.chk_server_submit.argtypes = [vp, u64, vp, vp, vp, C.POINTER(u64)]
+    L.chk_server_wait.argtypes = [vp, u64, vp, vp, vp, vp, vp, dp_]
+    L.chk_server_stats.argtypes = [vp, C.POINTER(u64), dp_, dp_, dp_, dp_, dp_]
+    L.chk_server_reset_stats.argtypes = [vp]
+    L.chk_server_free.argtypes = [vp]
 
 
 def _check(code):
@@ -259,6 +266,58 @@ class Query:
     def close(self):
         if getattr(self, "_h", None):
             lib().chk_query_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over an index: submit() query files from any thread,
+    wait() for each result row; stats() gives latency percentiles."""
+
+    def __init__(self, index, slots=2, max_batch_files=64, hbm_budget=0, pvalues=False):
+        h = C.c_void_p()
+        _check(lib().chk_server_create(index.handle, slots, max_batch_files, hbm_budget,
+                                       1 if pvalues else 0, C.byref(h)))
+        self._h, self._index, self.pvalues = h, index, pvalues
+
+    def submit(self, shard, start, end):
+        shard, start, end = _u32(shard), _u32(start), _u32(end)
+        t = C.c_uint64()
+        _check(lib().chk_server_submit(self._h, len(start), _ptr(shard), _ptr(start), _ptr(end),
+                                       C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        F = self._index.nfiles
+        out = {"counts": np.zeros(F, np.uint64), "bp": np.zeros(F, np.uint64),
+               "p": np.zeros(F, np.float64), "llr": np.zeros(F, np.float64),
+               "flags": np.zeros(F, np.uint8)}
+        lat = C.c_double()
+        _check(lib().chk_server_wait(self._h, ticket,
+                                     *[out[k].ctypes.data_as(C.c_void_p)
+                                       for k in ("counts", "bp", "p", "llr", "flags")],
+                                     C.byref(lat)))
+        out["latency_ms"] = lat.value
+        return out
+
+    def stats(self):
+        n = C.c_uint64()
+        v = [C.c_double() for _ in range(5)]
+        _check(lib().chk_server_stats(self._h, C.byref(n), *[C.byref(x) for x in v]))
+        keys = ("p50_ms", "p99_ms", "mean_ms", "swept_per_s", "files_per_batch")
+        return {"completed": n.value, **{k: x.value for k, x in zip(keys, v)}}
+
+    def reset_stats(self):
+        _check(lib().chk_server_reset_stats(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().chk_server_free(self._h)
             self._h = None
 
     def __del__(self):

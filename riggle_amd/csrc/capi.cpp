@@ -12,6 +12,10 @@ struct chk_index {
 struct chk_query {
     chk::Query q;
 };
+struct chk_server {
+    chk_index* idx = nullptr;
+    chk::Server* s = nullptr;
+};
 struct chk_graph {
     chk_index* idx = nullptr;
     chk_query* q = nullptr;
@@ -137,7 +141,8 @@ int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t
         try {
             h->q.idx = &idx->ix;
             h->q.nfiles = nfiles;
-            h->q.n = n;
+            h->q.n_in = n;
+            h->q.stream = idx->ix.stream;
             chk::query_upload(h->q, shard, start, end, qfile);
             chk::query_groups(h->q, shard, start, end, qfile);
         } catch (...) {
@@ -458,6 +463,65 @@ int chk_graph_free(chk_graph* g) {
         set_device(g->idx->ix.device);
         (void)hipStreamSynchronize(g->idx->ix.stream);
         delete g;
+    });
+}
+
+int chk_server_create(chk_index* idx, uint32_t slots, uint32_t max_batch_files,
+                      uint64_t hbm_budget, int pvalues, chk_server** out) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && out, CHK_ERR_INVALID, "NULL argument");
+        *out = nullptr;
+        set_device(idx->ix.device);
+        auto* h = new chk_server();
+        try {
+            h->idx = idx;
+            h->s = chk::server_create(idx->ix, slots, max_batch_files, hbm_budget, pvalues != 0);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int chk_server_submit(chk_server* s, uint64_t n, const uint32_t* shard, const uint32_t* start,
+                      const uint32_t* end, uint64_t* ticket) {
+    return guarded([&] {
+        CHK_REQUIRE(s && s->s && ticket, CHK_ERR_INVALID, "NULL argument");
+        CHK_REQUIRE(n == 0 || (shard && start && end), CHK_ERR_INVALID, "interval arrays must be non-NULL");
+        *ticket = chk::server_submit(*s->s, n, shard, start, end);
+    });
+}
+
+int chk_server_wait(chk_server* s, uint64_t ticket, uint64_t* counts, uint64_t* bp, double* p,
+                    double* llr, uint8_t* flags, double* latency_ms) {
+    return guarded([&] {
+        CHK_REQUIRE(s && s->s, CHK_ERR_INVALID, "NULL server");
+        chk::server_wait(*s->s, ticket, counts, bp, p, llr, flags, latency_ms);
+    });
+}
+
+int chk_server_stats(chk_server* s, uint64_t* completed, double* p50_ms, double* p99_ms,
+                     double* mean_ms, double* swept_per_s, double* files_per_batch) {
+    return guarded([&] {
+        CHK_REQUIRE(s && s->s, CHK_ERR_INVALID, "NULL server");
+        chk::server_stats(*s->s, completed, p50_ms, p99_ms, mean_ms, swept_per_s, files_per_batch);
+    });
+}
+
+int chk_server_reset_stats(chk_server* s) {
+    return guarded([&] {
+        CHK_REQUIRE(s && s->s, CHK_ERR_INVALID, "NULL server");
+        chk::server_reset_stats(*s->s);
+    });
+}
+
+int chk_server_free(chk_server* s) {
+    return guarded([&] {
+        if (!s) return;
+        set_device(s->idx->ix.device);
+        chk::server_free(s->s);
+        delete s;
     });
 }
 

@@ -67,6 +67,32 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Pinned host allocation (hipHostMalloc'd); grows on demand, never shrinks.
+template <typename T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    void reserve(size_t count) {
+        if (count <= n) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipHostMalloc((void**)&p, (count ? count : 1) * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw Error(CHK_ERR_NOMEM, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+        }
+        n = count;
+    }
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 inline int num_cus(int device) {
     hipDeviceProp_t prop;
     CHK_HIP(hipGetDeviceProperties(&prop, device));

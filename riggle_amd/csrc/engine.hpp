@@ -115,6 +115,8 @@ struct Index {
     DevBuf<int32_t> s_coord, e_coord;
     DevBuf<uint16_t> s_lfid, e_lfid;
     uint32_t nlayers = 0;
+    uint64_t span = 0;                // sum over shards of the largest start
+    std::vector<uint64_t> layer_n;    // intervals per size layer (all shards)
     std::vector<LayerSeg> lseg;       // [nshards * nlayers]
     DevBuf<LayerSeg> d_lseg;
     DevBuf<int32_t> l_start, l_end;
@@ -145,6 +147,12 @@ struct Job {
     uint32_t part0;   // first merge-path boundary
 };
 
+// One query file's block in the canonical and the placed layouts.
+struct PlaceFile {
+    uint64_t src, dst;            // first interval in the canonical / placed arrays
+    uint32_t n, row, drank, pad;  // drank = rank among the dense files, or ~0
+};
+
 // Sparse work item: query intervals [begin, end) (upload order) of one file.
 struct SparseItem {
     uint32_t row, begin, end, pad;
@@ -153,9 +161,16 @@ struct SparseItem {
 struct Query {
     Index* idx = nullptr;
     uint32_t nfiles = 0;
+    hipStream_t stream = nullptr;     // uploads and plan copies (the index's stream by default)
+    uint64_t n_in = 0;                // intervals given
     uint64_t n = 0;                   // intervals kept (on shards of the DB)
-    std::vector<uint32_t> h_shard, h_start, h_end, h_row;  // host copy, input order
-    bool placed = false;
+    std::vector<double> file_len;     // total length per query file
+    std::vector<uint64_t> fs_n;       // [nfiles * nshards] intervals per (file, shard)
+    std::vector<uint64_t> file_off;   // canonical layout: first interval of each file
+    PinnedBuf<uint32_t> st_shard, st_start, st_end;  // canonical layout, host staging
+    DevBuf<uint32_t> c_shard, c_start, c_end;         // canonical layout on the device
+    DevBuf<PlaceFile> place;
+    bool staged = false, placed = false;
     uint64_t n_dense = 0;             // the first n_dense belong to dense files
     std::vector<uint64_t> file_n;     // intervals per query file
     std::vector<uint8_t> file_dense;  // method per query file
@@ -209,6 +224,10 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
                  const uint32_t* fid);
 void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                   const uint32_t* qfile);
+// Canonical layout (grouped by file) in the staging buffers -> device, then plan.
+void query_stage_commit(Query& q);
+// Whether a query file of n intervals and total length takes the dense path.
+bool file_is_dense(const Index& ix, uint64_t n, double total_len);
 // Method per file and device placement for the current tuning.
 void query_place(Query& q);
 // Placement plus the host work plan of the dense and sparse paths for the
@@ -225,6 +244,18 @@ double time_sweeps(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps);
 // Coordinate-descent autotuner over the sweep knobs for this batch; leaves the
 // best values in the index's tuning and returns the best time per sweep.
 double autotune(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps, std::string* report);
+// Serving form (server.cpp).
+struct Server;
+Server* server_create(Index& ix, uint32_t slots, uint32_t max_batch_files, uint64_t hbm_budget,
+                      bool pvalues);
+void server_free(Server* s);
+uint64_t server_submit(Server& s, uint64_t n, const uint32_t* shard, const uint32_t* start,
+                       const uint32_t* end);
+void server_wait(Server& s, uint64_t ticket, uint64_t* counts, uint64_t* bp, double* p, double* llr,
+                 uint8_t* flags, double* latency_ms);
+void server_stats(Server& s, uint64_t* completed, double* p50, double* p99, double* mean,
+                  double* swept_per_s, double* files_per_batch);
+void server_reset_stats(Server& s);
 // Plain copy of `bytes` (reads and writes that many bytes once).
 void calibrate_copy(Index& ix, uint64_t bytes);
 

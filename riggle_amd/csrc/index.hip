@@ -183,6 +183,16 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
         lmax[b] = std::max(lmax[b], start[i]);
     }
     ix.lseg.assign((size_t)S * NL, LayerSeg{});
+    ix.layer_n.assign(NL, 0);
+    ix.span = 0;
+    for (uint32_t sh = 0; sh < S; sh++) {
+        uint32_t mx = 0;
+        for (uint32_t k = 0; k < NL; k++) {
+            ix.layer_n[k] += lcount[(size_t)sh * NL + k];
+            mx = std::max(mx, lmax[(size_t)sh * NL + k]);
+        }
+        ix.span += mx;
+    }
     uint64_t off = 0, jto = 0;
     for (size_t b = 0; b < (size_t)S * NL; b++) {
         LayerSeg& L = ix.lseg[b];
@@ -252,19 +262,22 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
     ix.scratch.vals_alt.release();
 }
 
-// Take one batch of query files.  Intervals on shards absent from the DB are
-// dropped (the reference skips such shards, src/tasks/query/mod.rs:193-199);
-// the rest are kept on the host and placed on the device by query_place.
+// Query ingest.  A batch is first laid out canonically -- intervals grouped
+// by query file in file order, those on shards absent from the DB dropped
+// (the reference skips such shards, src/tasks/query/mod.rs:193-199) -- in
+// pinned staging memory and copied to the device once.  query_place then
+// orders the files for the sweep on the device (gather kernel), so a change
+// of method threshold never goes back to the host.
+
+// Canonical layout from arbitrary per-interval file ids (counting sort).
 void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                   const uint32_t* qfile) {
     Index& ix = *q.idx;
-    const uint64_t n_in = q.n;
+    const uint64_t n_in = q.n_in;
     const uint32_t S = ix.nshards, R = q.nfiles;
     q.file_n.assign(R, 0);
-    q.h_shard.clear();
-    q.h_start.clear();
-    q.h_end.clear();
-    q.h_row.clear();
+    q.file_len.assign(R, 0.0);
+    q.fs_n.assign((size_t)R * S, 0);
     for (uint64_t i = 0; i < n_in; i++) {
         CHK_REQUIRE(start[i] < end[i], CHK_ERR_INVALID, "query interval start must be less than end");
         CHK_REQUIRE(end[i] <= 0x7FFFFFFFu, CHK_ERR_INVALID,
@@ -273,85 +286,157 @@ void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const 
         CHK_REQUIRE(r < R, CHK_ERR_INVALID, "query file id out of range");
         if (shard[i] >= S || !ix.shard_n[shard[i]]) continue;
         q.file_n[r]++;
-        q.h_shard.push_back(shard[i]);
-        q.h_start.push_back(start[i]);
-        q.h_end.push_back(end[i]);
-        q.h_row.push_back(r);
+        q.file_len[r] += (double)(end[i] - start[i]);
+        q.fs_n[(size_t)r * S + shard[i]]++;
     }
-    q.n = q.h_start.size();
+    q.file_off.assign(R + 1, 0);
+    for (uint32_t r = 0; r < R; r++) q.file_off[r + 1] = q.file_off[r] + q.file_n[r];
+    q.n = q.file_off[R];
     CHK_REQUIRE(q.n < (1ull << 32), CHK_ERR_INVALID, "query batch exceeds 2^32 intervals");
+    q.st_shard.reserve(q.n);
+    q.st_start.reserve(q.n);
+    q.st_end.reserve(q.n);
+    std::vector<uint64_t> fill(q.file_off.begin(), q.file_off.end() - 1);
+    for (uint64_t i = 0; i < n_in; i++) {
+        if (shard[i] >= S || !ix.shard_n[shard[i]]) continue;
+        const uint64_t k = fill[qfile ? qfile[i] : 0u]++;
+        q.st_shard.p[k] = shard[i];
+        q.st_start.p[k] = start[i];
+        q.st_end.p[k] = end[i];
+    }
+    query_stage_commit(q);
+}
+
+// Canonical layout already in the staging buffers (file_n, file_len, fs_n
+// and file_off filled by the caller): copy it to the device.
+void query_stage_commit(Query& q) {
+    const uint64_t n = q.n;
+    q.c_shard.reserve(n);
+    q.c_start.reserve(n);
+    q.c_end.reserve(n);
+    if (n) {
+        CHK_HIP(hipMemcpyAsync(q.c_shard.p, q.st_shard.p, n * 4, hipMemcpyHostToDevice, q.stream));
+        CHK_HIP(hipMemcpyAsync(q.c_start.p, q.st_start.p, n * 4, hipMemcpyHostToDevice, q.stream));
+        CHK_HIP(hipMemcpyAsync(q.c_end.p, q.st_end.p, n * 4, hipMemcpyHostToDevice, q.stream));
+    }
+    // the staging buffers are reused by the next batch of this query object:
+    // the copies must have read them before the caller refills them
+    q.staged = true;
     q.placed = false;
     q.plan_version = 0xFFFFFFFFu;
     query_plan(q);
 }
 
-// Method per file and device placement: the sparse path enumerates pairs
-// through the layered table, the dense path streams the whole DB, so a file
-// goes sparse when it is small against the DB.  Intervals are grouped by
-// file, dense files first, so each sparse file is a contiguous range and the
-// dense files' events are a prefix.  Redone only when the choice changes.
+namespace {
+
+// placed[dst + k] = canonical[src + k] for every file; dense files' intervals
+// also get their (dense file, shard) segment id for the event keys.
+__global__ void place_kernel(const PlaceFile* files, uint32_t nfiles, uint64_t n, uint32_t S,
+                             const uint32_t* cs, const uint32_t* cb, const uint32_t* ce,
+                             uint32_t* shard, uint32_t* start, uint32_t* end, uint32_t* row,
+                             uint32_t* segid) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nfiles;  // last file with dst <= i (among non-empty ones)
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (files[mid].dst <= i) lo = mid; else hi = mid;
+        }
+        const PlaceFile f = files[lo];
+        const uint64_t k = f.src + (i - f.dst);
+        const uint32_t sh = cs[k];
+        shard[i] = sh;
+        start[i] = cb[k];
+        end[i] = ce[k];
+        row[i] = f.row;
+        if (f.drank != 0xFFFFFFFFu) segid[i] = f.drank * S + sh;
+    }
+}
+
+}  // namespace
+
+// Method per file and device placement.  The dense path streams the whole DB
+// (2 |D| elements whatever the file's size); the sparse path visits, per
+// query interval and size layer K, the DB starts within lookback_K + its
+// length before its end -- about N_K (lookback_K + len) / span of them.  A
+// file goes sparse when SPARSE_COST times its expected visits is below the
+// dense pass (SPARSE_COST: the sparse path's cost per visited element against
+// the dense path's, with headroom for DB hotspots the query shares), or, with
+// an explicit sweep.sparse_max, when it has at most that many intervals.
+// Files are ordered dense first, so each sparse file is a contiguous range
+// and the dense files' events are a prefix.  Redone only when the choice
+// changes.
+constexpr double SPARSE_COST = 8.0;
+
+bool file_is_dense(const Index& ix, uint64_t n, double total_len) {
+    if (!n) return false;
+    if (ix.tune.method == 1) return true;
+    if (ix.tune.method == 2) return false;
+    if (ix.tune.sparse_max) return n > ix.tune.sparse_max;
+    double lb_sum = 0.0, n_sum = 0.0;  // sum_K N_K lookback_K, sum_K N_K
+    for (uint32_t k = 0; k < ix.nlayers; k++) {
+        lb_sum += (double)ix.layer_n[k] * (double)ix.lseg[k].lookback;
+        n_sum += (double)ix.layer_n[k];
+    }
+    const double span = (double)std::max<uint64_t>(ix.span, 1);
+    const double visits = ((double)n * lb_sum + total_len * n_sum) / span;
+    return SPARSE_COST * visits > 2.0 * (double)ix.n + 2.0 * (double)n;
+}
+
 void query_place(Query& q) {
     Index& ix = *q.idx;
     const uint32_t S = ix.nshards, R = q.nfiles;
-    const uint64_t smax = ix.tune.sparse_max ? ix.tune.sparse_max : std::max<uint64_t>(1, ix.n / 1024);
     std::vector<uint8_t> dense(R, 0);
-    for (uint32_t r = 0; r < R; r++)
-        dense[r] = q.file_n[r] &&
-                   (ix.tune.method == 1 || (ix.tune.method == 0 && q.file_n[r] > smax));
+    for (uint32_t r = 0; r < R; r++) dense[r] = file_is_dense(ix, q.file_n[r], q.file_len[r]);
     if (q.placed && dense == q.file_dense) return;
     q.file_dense = dense;
     q.dense_rows.clear();
-    std::vector<uint32_t> drank(R, 0xFFFFFFFFu);
-    for (uint32_t r = 0; r < R; r++)
-        if (dense[r]) {
-            drank[r] = (uint32_t)q.dense_rows.size();
-            q.dense_rows.push_back(r);
-        }
-    const uint32_t ND = (uint32_t)q.dense_rows.size();
-    std::vector<uint64_t> pos(R, 0);
+    std::vector<PlaceFile> pf;
     uint64_t p = 0;
-    for (uint32_t r = 0; r < R; r++)
-        if (dense[r]) { pos[r] = p; p += q.file_n[r]; }
-    q.n_dense = p;
-    for (uint32_t r = 0; r < R; r++)
-        if (!dense[r]) { pos[r] = p; p += q.file_n[r]; }
+    for (int pass = 0; pass < 2; pass++)
+        for (uint32_t r = 0; r < R; r++) {
+            if ((dense[r] != 0) != (pass == 0) || !q.file_n[r]) continue;
+            PlaceFile f{};
+            f.src = q.file_off[r];
+            f.dst = p;
+            f.n = (uint32_t)q.file_n[r];
+            f.row = r;
+            f.drank = pass == 0 ? (uint32_t)q.dense_rows.size() : 0xFFFFFFFFu;
+            if (pass == 0) q.dense_rows.push_back(r);
+            pf.push_back(f);
+            p += q.file_n[r];
+            if (pass == 0) q.n_dense = p;
+        }
+    if (q.dense_rows.empty()) q.n_dense = 0;
     CHK_REQUIRE(2 * q.n_dense < (1ull << 32) - 1, CHK_ERR_INVALID,
                 "query batch too large (2 x dense intervals must fit u32)");
-    const uint64_t n = q.n;
-    std::vector<uint32_t> hs(n), hb(n), he(n), hr(n), hg(q.n_dense);
+    const uint32_t ND = (uint32_t)q.dense_rows.size();
     q.seg_n.assign((size_t)ND * S, 0);
-    for (uint64_t i = 0; i < n; i++) {
-        const uint32_t r = q.h_row[i], s = q.h_shard[i];
-        const uint64_t k = pos[r]++;
-        hs[k] = s;
-        hb[k] = q.h_start[i];
-        he[k] = q.h_end[i];
-        hr[k] = r;
-        if (dense[r]) {
-            hg[k] = drank[r] * S + s;
-            q.seg_n[(size_t)drank[r] * S + s]++;
-        }
-    }
-    hipStream_t st = ix.stream;
+    for (uint32_t k = 0; k < ND; k++)
+        for (uint32_t s = 0; s < S; s++) q.seg_n[(size_t)k * S + s] = q.fs_n[(size_t)q.dense_rows[k] * S + s];
+    const uint64_t n = q.n;
     q.shard.reserve(n);
     q.start.reserve(n);
     q.end.reserve(n);
     q.row.reserve(n);
     q.segid.reserve(q.n_dense);
+    q.place.reserve(pf.size());
     if (n) {
-        CHK_HIP(hipMemcpyAsync(q.shard.p, hs.data(), n * 4, hipMemcpyHostToDevice, st));
-        CHK_HIP(hipMemcpyAsync(q.start.p, hb.data(), n * 4, hipMemcpyHostToDevice, st));
-        CHK_HIP(hipMemcpyAsync(q.end.p, he.data(), n * 4, hipMemcpyHostToDevice, st));
-        CHK_HIP(hipMemcpyAsync(q.row.p, hr.data(), n * 4, hipMemcpyHostToDevice, st));
+        CHK_HIP(hipMemcpyAsync(q.place.p, pf.data(), pf.size() * sizeof(PlaceFile),
+                               hipMemcpyHostToDevice, q.stream));
+        const unsigned g = (unsigned)std::min<uint64_t>(nblk(n), (uint64_t)ix.ncu * 16);
+        hipLaunchKernelGGL(place_kernel, dim3(g), dim3(NT), 0, q.stream,
+                           q.place.p, (uint32_t)pf.size(), n, S,
+                           q.c_shard.p, q.c_start.p, q.c_end.p, q.shard.p, q.start.p, q.end.p,
+                           q.row.p, q.segid.p);
+        CHK_HIP(hipGetLastError());
     }
-    if (q.n_dense)
-        CHK_HIP(hipMemcpyAsync(q.segid.p, hg.data(), q.n_dense * 4, hipMemcpyHostToDevice, st));
     const size_t m = 2 * q.n_dense;
     q.keys.reserve(m);
     q.key32.reserve(m + 1);
     q.rs.reserve(m + 1);
     q.w.reserve(m + 1);
-    CHK_HIP(hipStreamSynchronize(st));
+    CHK_HIP(hipStreamSynchronize(q.stream));  // pf is host memory of this frame
     q.placed = true;
 }
 

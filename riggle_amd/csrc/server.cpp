@@ -1,0 +1,431 @@
+// server.cpp -- serving form: a request loop over an HBM-resident index with
+// admission by HBM footprint and continuous batching.
+//
+// Clients submit query files (one file per request) from any thread.  One
+// scheduler thread owns the device work.  Whenever a batch slot is free it
+// takes the waiting requests in arrival order, as many as the HBM budget and
+// the per-batch file limit allow, places them as one multi-file query batch
+// and enqueues the sweep (and the p-values) on that slot's own stream, so
+// batches admitted while others run overlap on the device.  Completed slots
+// are found by polling their events; each request's row of counts, bp and
+// p-values is copied out and its waiter woken.  Slots keep their device and
+// pinned host buffers between batches, so the steady state allocates
+// nothing.  A request's row depends only on its own intervals and the index,
+// never on the batch it shared, so results are identical alone and in a mix.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "engine.hpp"
+
+namespace chk {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    uint64_t id = 0;
+    std::vector<uint32_t> shard, start, end;  // on shards of the DB only
+    std::vector<uint64_t> fs_n;               // intervals per shard
+    double total_len = 0.0;
+    uint64_t n_given = 0;
+    uint64_t footprint = 0;
+    Clock::time_point t_submit, t_done;
+    std::vector<uint64_t> counts, bp;
+    std::vector<double> p, llr;
+    std::vector<uint8_t> flags;
+    bool done = false;
+    std::string error;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::unique_ptr<Query> q;
+    DevBuf<uint64_t> dc, db;
+    DevBuf<double> dp, dl;
+    DevBuf<uint8_t> df;
+    PinnedBuf<uint64_t> hc, hb;
+    PinnedBuf<double> hp, hl;
+    PinnedBuf<uint8_t> hf;
+    std::vector<Request*> reqs;
+    uint64_t footprint = 0;
+    bool busy = false;
+};
+
+struct ServerOpts {
+    uint32_t slots = 2;
+    uint32_t max_batch_files = 64;
+    uint64_t hbm_budget = 0;
+    bool pvalues = false;
+};
+
+struct Server {
+    Index* ix = nullptr;
+    ServerOpts opt;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    uint64_t next_id = 1, inflight_bytes = 0;
+    bool stop = false;
+    std::string fatal;
+    // statistics over completed requests
+    std::vector<double> lat_ms;
+    uint64_t swept = 0, batches = 0, batched_files = 0;
+    bool have_first = false;
+    Clock::time_point first_submit, last_done;
+    std::thread th;
+
+    // device bytes a request needs while in flight: uploaded intervals and
+    // events, the event sort's scratch, its merge-path partition (dense
+    // files) and its output row
+    uint64_t footprint(uint64_t n) const {
+        const uint64_t per_iv = 20 + 2 * (8 + 4 + 4 + 8) + 2 * 12;
+        const uint64_t part = (2 * ix->n + 2 * n) / 1024 * 4 + 4096;
+        const uint64_t row = (uint64_t)ix->nfiles * (8 + 8 + 8 + 8 + 1);
+        return per_iv * n + part + row;
+    }
+
+    void start() {
+        for (uint32_t i = 0; i < opt.slots; i++) {
+            auto s = std::make_unique<Slot>();
+            CHK_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+            CHK_HIP(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
+            s->q = std::make_unique<Query>();
+            s->q->idx = ix;
+            s->q->stream = s->stream;
+            slots.push_back(std::move(s));
+        }
+        th = std::thread([this] { loop(); });
+    }
+
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        if (th.joinable()) th.join();
+        for (auto& s : slots) {
+            if (s->stream) (void)hipStreamSynchronize(s->stream);
+            if (s->done) (void)hipEventDestroy(s->done);
+            if (s->stream) (void)hipStreamDestroy(s->stream);
+        }
+    }
+
+    // Launch one batch on a free slot (scheduler thread, lock not held): the
+    // requests' intervals go straight into the slot query's pinned staging
+    // as its canonical layout (one file per request).
+    void launch(Slot& s) {
+        const uint32_t B = (uint32_t)s.reqs.size(), S = ix->nshards;
+        Query& q = *s.q;
+        q.nfiles = B;
+        q.file_n.assign(B, 0);
+        q.file_len.assign(B, 0.0);
+        q.fs_n.assign((size_t)B * S, 0);
+        q.file_off.assign(B + 1, 0);
+        for (uint32_t b = 0; b < B; b++) {
+            const Request* r = s.reqs[b];
+            q.file_n[b] = r->start.size();
+            q.file_len[b] = r->total_len;
+            std::copy(r->fs_n.begin(), r->fs_n.end(), q.fs_n.begin() + (size_t)b * S);
+            q.file_off[b + 1] = q.file_off[b] + r->start.size();
+        }
+        const uint64_t n = q.file_off[B];
+        q.n = q.n_in = n;
+        q.st_shard.reserve(n);
+        q.st_start.reserve(n);
+        q.st_end.reserve(n);
+        for (uint32_t b = 0; b < B; b++) {
+            const Request* r = s.reqs[b];
+            const uint64_t o = q.file_off[b], m = r->start.size();
+            std::memcpy(q.st_shard.p + o, r->shard.data(), m * 4);
+            std::memcpy(q.st_start.p + o, r->start.data(), m * 4);
+            std::memcpy(q.st_end.p + o, r->end.data(), m * 4);
+        }
+        query_stage_commit(q);
+        if (opt.pvalues) {
+            std::vector<uint32_t> qf(n);
+            for (uint32_t b = 0; b < B; b++)
+                std::fill(qf.begin() + q.file_off[b], qf.begin() + q.file_off[b + 1], b);
+            query_groups(q, q.st_shard.p, q.st_start.p, q.st_end.p, qf.data());
+        }
+        const size_t cells = (size_t)B * ix->nfiles;
+        s.dc.reserve(cells);
+        s.db.reserve(cells);
+        s.hc.reserve(cells);
+        s.hb.reserve(cells);
+        sweep_run(q, s.dc.p, s.db.p, s.stream);
+        CHK_HIP(hipMemcpyAsync(s.hc.p, s.dc.p, cells * 8, hipMemcpyDeviceToHost, s.stream));
+        CHK_HIP(hipMemcpyAsync(s.hb.p, s.db.p, cells * 8, hipMemcpyDeviceToHost, s.stream));
+        if (opt.pvalues) {
+            s.dp.reserve(cells);
+            s.dl.reserve(cells);
+            s.df.reserve(cells);
+            s.hp.reserve(cells);
+            s.hl.reserve(cells);
+            s.hf.reserve(cells);
+            pvalues_run(q, s.dc.p, s.db.p, s.dp.p, s.dl.p, s.df.p, s.stream);
+            CHK_HIP(hipMemcpyAsync(s.hp.p, s.dp.p, cells * 8, hipMemcpyDeviceToHost, s.stream));
+            CHK_HIP(hipMemcpyAsync(s.hl.p, s.dl.p, cells * 8, hipMemcpyDeviceToHost, s.stream));
+            CHK_HIP(hipMemcpyAsync(s.hf.p, s.df.p, cells, hipMemcpyDeviceToHost, s.stream));
+        }
+        CHK_HIP(hipEventRecord(s.done, s.stream));
+    }
+
+    // Copy each request's row out of a completed slot (lock held).
+    void complete(Slot& s, const std::string& err) {
+        const uint32_t F = ix->nfiles;
+        const auto now = Clock::now();
+        for (size_t b = 0; b < s.reqs.size(); b++) {
+            Request* r = s.reqs[b];
+            if (err.empty()) {
+                r->counts.assign(s.hc.p + b * F, s.hc.p + (b + 1) * F);
+                r->bp.assign(s.hb.p + b * F, s.hb.p + (b + 1) * F);
+                if (opt.pvalues) {
+                    r->p.assign(s.hp.p + b * F, s.hp.p + (b + 1) * F);
+                    r->llr.assign(s.hl.p + b * F, s.hl.p + (b + 1) * F);
+                    r->flags.assign(s.hf.p + b * F, s.hf.p + (b + 1) * F);
+                }
+                lat_ms.push_back(std::chrono::duration<double, std::milli>(now - r->t_submit).count());
+                swept += r->n_given + ix->n;
+            } else {
+                r->error = err;
+            }
+            r->t_done = now;
+            r->done = true;
+            std::vector<uint32_t>().swap(r->shard);
+            std::vector<uint32_t>().swap(r->start);
+            std::vector<uint32_t>().swap(r->end);
+        }
+        last_done = now;
+        inflight_bytes -= s.footprint;
+        s.footprint = 0;
+        s.reqs.clear();
+        s.busy = false;
+    }
+
+    void loop() {
+        try {
+            CHK_HIP(hipSetDevice(ix->device));
+        } catch (const Error& e) {
+            std::lock_guard<std::mutex> g(mu);
+            fatal = e.what();
+            return;
+        }
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            bool any_busy = false;
+            for (auto& s : slots) any_busy |= s->busy;
+            if (stop && !any_busy) break;
+            // completions
+            for (auto& s : slots) {
+                if (!s->busy) continue;
+                const hipError_t e = hipEventQuery(s->done);
+                if (e == hipSuccess) {
+                    complete(*s, "");
+                } else if (e != hipErrorNotReady) {
+                    (void)hipGetLastError();
+                    complete(*s, std::string("device error: ") + hipGetErrorString(e));
+                }
+            }
+            cv_done.notify_all();
+            // admission: fill every free slot from the head of the queue
+            for (auto& s : slots) {
+                if (s->busy || pending.empty() || stop) continue;
+                uint64_t fp = 0;
+                while (!pending.empty() && s->reqs.size() < opt.max_batch_files) {
+                    Request* r = pending.front();
+                    const bool fits = inflight_bytes + fp + r->footprint <= opt.hbm_budget;
+                    // a request larger than the whole budget runs alone
+                    const bool alone = inflight_bytes == 0 && s->reqs.empty();
+                    if (!fits && !alone) break;
+                    fp += r->footprint;
+                    s->reqs.push_back(r);
+                    pending.pop_front();
+                }
+                if (s->reqs.empty()) continue;
+                s->busy = true;
+                s->footprint = fp;
+                inflight_bytes += fp;
+                batches++;
+                batched_files += s->reqs.size();
+                lk.unlock();
+                std::string err;
+                try {
+                    launch(*s);
+                } catch (const std::exception& e) {
+                    err = e.what();
+                }
+                lk.lock();
+                if (!err.empty()) {
+                    (void)hipStreamSynchronize(s->stream);
+                    complete(*s, err);
+                    cv_done.notify_all();
+                }
+            }
+            any_busy = false;
+            for (auto& s : slots) any_busy |= s->busy;
+            if (any_busy) {
+                // poll the device: short sleeps keep the completion latency low
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+                lk.lock();
+            } else {
+                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+            }
+        }
+    }
+
+    // Validate and keep one query file (client thread): intervals on shards
+    // absent from the DB are dropped here, as by query_upload.
+    uint64_t submit(uint64_t n, const uint32_t* shard, const uint32_t* start, const uint32_t* end) {
+        auto r = std::make_unique<Request>();
+        const uint32_t S = ix->nshards;
+        r->n_given = n;
+        r->fs_n.assign(S, 0);
+        uint64_t kept = 0;
+        for (uint64_t i = 0; i < n; i++) {
+            CHK_REQUIRE(start[i] < end[i], CHK_ERR_INVALID, "query interval start must be less than end");
+            CHK_REQUIRE(end[i] <= 0x7FFFFFFFu, CHK_ERR_INVALID,
+                        "query coordinate exceeds 2^31-1 (device coordinates are int32)");
+            kept += shard[i] < S && ix->shard_n[shard[i]] ? 1 : 0;
+        }
+        r->shard.resize(kept);
+        r->start.resize(kept);
+        r->end.resize(kept);
+        uint64_t k = 0;
+        for (uint64_t i = 0; i < n; i++) {
+            if (shard[i] >= S || !ix->shard_n[shard[i]]) continue;
+            r->shard[k] = shard[i];
+            r->start[k] = start[i];
+            r->end[k] = end[i];
+            r->fs_n[shard[i]]++;
+            r->total_len += (double)(end[i] - start[i]);
+            k++;
+        }
+        r->footprint = footprint(kept);
+        std::lock_guard<std::mutex> g(mu);
+        CHK_REQUIRE(fatal.empty(), CHK_ERR_HIP, "server thread failed: " + fatal);
+        r->id = next_id++;
+        r->t_submit = Clock::now();
+        if (!have_first) {
+            have_first = true;
+            first_submit = r->t_submit;
+        }
+        Request* raw = r.get();
+        reqs[raw->id] = std::move(r);
+        pending.push_back(raw);
+        cv_work.notify_one();
+        return raw->id;
+    }
+
+    std::unique_ptr<Request> wait(uint64_t id) {
+        std::unique_lock<std::mutex> lk(mu);
+        auto it = reqs.find(id);
+        CHK_REQUIRE(it != reqs.end(), CHK_ERR_NOT_FOUND, "unknown or already collected ticket");
+        Request* r = it->second.get();
+        cv_done.wait(lk, [&] { return r->done || !fatal.empty(); });
+        CHK_REQUIRE(r->done, CHK_ERR_HIP, "server thread failed: " + fatal);
+        std::unique_ptr<Request> out = std::move(it->second);
+        reqs.erase(it);
+        return out;
+    }
+};
+
+Server* server_create(Index& ix, uint32_t slots, uint32_t max_batch_files, uint64_t hbm_budget,
+                      bool pvalues) {
+    CHK_REQUIRE(slots >= 1 && slots <= 8, CHK_ERR_INVALID, "slots must be 1..8");
+    CHK_REQUIRE(max_batch_files >= 1, CHK_ERR_INVALID, "max_batch_files must be positive");
+    if (pvalues) {
+        CHK_REQUIRE(ix.mom.present, CHK_ERR_NOT_FOUND, "p-values need the index's moment tables");
+        CHK_REQUIRE(ix.mom.genome, CHK_ERR_INVALID, "p-values need the genome table");
+    }
+    auto* s = new Server();
+    s->ix = &ix;
+    s->opt.slots = slots;
+    s->opt.max_batch_files = max_batch_files;
+    s->opt.pvalues = pvalues;
+    if (!hbm_budget) {
+        size_t fr = 0, tot = 0;
+        CHK_HIP(hipMemGetInfo(&fr, &tot));
+        hbm_budget = fr / 2;  // the other half stays for the slots' reuse and the runtime
+    }
+    s->opt.hbm_budget = hbm_budget;
+    try {
+        s->start();
+    } catch (...) {
+        delete s;
+        throw;
+    }
+    return s;
+}
+
+void server_free(Server* s) { delete s; }
+
+uint64_t server_submit(Server& s, uint64_t n, const uint32_t* shard, const uint32_t* start,
+                       const uint32_t* end) {
+    return s.submit(n, shard, start, end);
+}
+
+void server_wait(Server& s, uint64_t ticket, uint64_t* counts, uint64_t* bp, double* p, double* llr,
+                 uint8_t* flags, double* latency_ms) {
+    std::unique_ptr<Request> r = s.wait(ticket);
+    CHK_REQUIRE(r->error.empty(), CHK_ERR_HIP, "request failed: " + r->error);
+    const uint32_t F = s.ix->nfiles;
+    if (counts) std::copy(r->counts.begin(), r->counts.end(), counts);
+    if (bp) std::copy(r->bp.begin(), r->bp.end(), bp);
+    if (s.opt.pvalues) {
+        if (p) std::copy(r->p.begin(), r->p.end(), p);
+        if (llr) std::copy(r->llr.begin(), r->llr.end(), llr);
+        if (flags) std::copy(r->flags.begin(), r->flags.end(), flags);
+    } else {
+        if (p) std::fill(p, p + F, 1.0);
+        if (llr) std::fill(llr, llr + F, 0.0);
+        if (flags) std::fill(flags, flags + F, (uint8_t)0);
+    }
+    if (latency_ms)
+        *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+}
+
+void server_stats(Server& s, uint64_t* completed, double* p50, double* p99, double* mean,
+                  double* swept_per_s, double* files_per_batch) {
+    std::lock_guard<std::mutex> g(s.mu);
+    std::vector<double> v = s.lat_ms;
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double q) {
+        if (v.empty()) return 0.0;
+        const size_t i = std::min(v.size() - 1, (size_t)std::ceil(q * (double)v.size()) - 1);
+        return v[i];
+    };
+    double sum = 0.0;
+    for (double x : v) sum += x;
+    if (completed) *completed = v.size();
+    if (p50) *p50 = pct(0.50);
+    if (p99) *p99 = pct(0.99);
+    if (mean) *mean = v.empty() ? 0.0 : sum / (double)v.size();
+    if (swept_per_s) {
+        const double secs =
+            s.have_first ? std::chrono::duration<double>(s.last_done - s.first_submit).count() : 0.0;
+        *swept_per_s = secs > 0 ? (double)s.swept / secs : 0.0;
+    }
+    if (files_per_batch) *files_per_batch = s.batches ? (double)s.batched_files / s.batches : 0.0;
+}
+
+void server_reset_stats(Server& s) {
+    std::lock_guard<std::mutex> g(s.mu);
+    s.lat_ms.clear();
+    s.swept = 0;
+    s.batches = 0;
+    s.batched_files = 0;
+    s.have_first = false;
+}
+
+}  // namespace chk

@@ -58,7 +58,7 @@ void query_groups(Query& q, const uint32_t* shard, const uint32_t* start, const 
     const uint32_t S = ix.nshards;
     const size_t nseg = (size_t)q.nfiles * S;
     std::vector<uint32_t> cnt(nseg + 1, 0);
-    for (uint64_t i = 0; i < q.n; i++) {
+    for (uint64_t i = 0; i < q.n_in; i++) {
         uint32_t c = shard[i];
         if (c >= S || ix.mom.nbins[c] == 0) continue;
         cnt[(size_t)(qfile ? qfile[i] : 0) * S + c + 1]++;
@@ -68,7 +68,7 @@ void query_groups(Query& q, const uint32_t* shard, const uint32_t* start, const 
     std::vector<uint32_t> len(cnt[nseg]);
     {
         std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
-        for (uint64_t i = 0; i < q.n; i++) {
+        for (uint64_t i = 0; i < q.n_in; i++) {
             uint32_t c = shard[i];
             if (c >= S || ix.mom.nbins[c] == 0) continue;
             size_t g = (size_t)(qfile ? qfile[i] : 0) * S + c;
@@ -110,7 +110,7 @@ void query_groups(Query& q, const uint32_t* shard, const uint32_t* start, const 
     }
     off[nseg] = (uint32_t)gl.size();
     q.ngroups = gl.size();
-    hipStream_t st = ix.stream;
+    hipStream_t st = q.stream;
     q.g_off.reserve(nseg + 1);
     CHK_HIP(hipMemcpyAsync(q.g_off.p, off.data(), (nseg + 1) * 4, hipMemcpyHostToDevice, st));
     if (q.ngroups) {

@@ -580,7 +580,7 @@ void query_plan(Query& q) {
             p += q.file_n[r];
         }
     }
-    hipStream_t st = ix.stream;
+    hipStream_t st = q.stream;
     q.jobs.reserve(J.size());
     q.chunk_job.reserve(CJ.size());
     q.part.reserve(std::max<uint32_t>(1, q.nbounds));
