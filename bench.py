@@ -128,7 +128,9 @@ def stats_leg(ix, qq, cfg, F, steps):
 
 def batched_leg(ix, cfg, F, nshards, D):
     """256 query files (log-uniform 100-2M intervals, 30 % hotspot) against the
-    same DB: one batched launch vs one plain launch per file."""
+    same DB.  Plain: one launch per file, each file on its own best path, then
+    dense-only (the plain merge-path method).  Batched: the autotuner picks the
+    knobs on the batch, then the batch runs as one captured HIP graph."""
     from riggle_amd import gen
     t = time.time()
     g = cfg["genome"]
@@ -140,8 +142,29 @@ def batched_leg(ix, cfg, F, nshards, D):
     off = np.concatenate([[0], np.cumsum(qc)])
     log(f"batched: generated 256 files, {int(qc.sum())} intervals in {time.time() - t:.1f}s")
     B = 256
+    d1, d2 = ix.alloc(8 * F), ix.alloc(8 * F)
+    singles = [ix.query(sh[off[i]:off[i + 1]], s[off[i]:off[i + 1]], e[off[i]:off[i + 1]])
+               for i in range(B)]
+
+    def plain_loop():
+        ix.sweep_async(singles[0], d1, d2)
+        ix.sync()
+        t = time.perf_counter()
+        for i in range(B):
+            ix.sweep_async(singles[i], d1, d2)
+        ix.sync()
+        return (time.perf_counter() - t) * 1e3
+
+    ms_plain = plain_loop()
+    ix.tune("sweep.method", 1)
+    ms_plain_dense = plain_loop()
+    ix.tune("sweep.method", 0)
     qb = ix.query(sh, s, e, qf, B)
     dc, db_ = ix.alloc(8 * B * F), ix.alloc(8 * B * F)
+    t = time.perf_counter()
+    best, knobs, report = ix.autotune(qb, dc, db_, reps=2)
+    t_tune = time.perf_counter() - t
+    log(f"batched: autotune {t_tune:.1f}s best {best:.2f} ms {knobs}")
     ix.sweep_async(qb, dc, db_)
     ix.sync()
     reps = 3
@@ -149,30 +172,33 @@ def batched_leg(ix, cfg, F, nshards, D):
     for _ in range(reps):
         ix.sweep_async(qb, dc, db_)
     ix.sync()
-    ms_batched = (time.perf_counter() - t) * 1e3 / reps
-    cb = dc.to_numpy(np.uint64, (B, F))
-    # plain: one query handle and launch per file
-    singles = [ix.query(sh[off[i]:off[i + 1]], s[off[i]:off[i + 1]], e[off[i]:off[i + 1]])
-               for i in range(B)]
-    d1, d2 = ix.alloc(8 * F), ix.alloc(8 * F)
-    ix.sweep_async(singles[0], d1, d2)
+    ms_eager = (time.perf_counter() - t) * 1e3 / reps
+    graph = ix.graph(qb, dc, db_)
+    graph.launch()
     ix.sync()
     t = time.perf_counter()
-    rows_ok = True
-    for i in range(B):
-        ix.sweep_async(singles[i], d1, d2)
+    for _ in range(reps):
+        graph.launch()
     ix.sync()
-    ms_plain = (time.perf_counter() - t) * 1e3
-    # spot-check rows of the batched matrix against single launches
+    ms_graph = (time.perf_counter() - t) * 1e3 / reps
+    cb = dc.to_numpy(np.uint64, (B, F))
+    rows_ok = True
     for i in (0, 1, B // 2, B - 1):
         ix.sweep_async(singles[i], d1, d2)
         rows_ok &= bool((d1.to_numpy(np.uint64, F) == cb[i]).all())
+    graph.close()
+    for k in ("sweep.sparse_max", "sweep.items", "sweep.chunk", "sweep.blocks_per_cu"):
+        ix.tune(k, 0)
     swept = B * D + int(qc.sum())
     return {"query_files": B, "query_intervals": int(qc.sum()), "intervals_swept": swept,
-            "batched_ms": ms_batched, "plain_ms_256_launches": ms_plain,
-            "batched_swept_per_s": swept / (ms_batched * 1e-3),
+            "batched_graph_ms": ms_graph, "batched_eager_ms": ms_eager,
+            "plain_ms_256_launches": ms_plain, "plain_dense_ms_256_launches": ms_plain_dense,
+            "batched_swept_per_s": swept / (ms_graph * 1e-3),
             "plain_swept_per_s": swept / (ms_plain * 1e-3),
-            "batched_over_plain": ms_plain / ms_batched, "rows_match_single_launches": rows_ok}
+            "plain_dense_swept_per_s": swept / (ms_plain_dense * 1e-3),
+            "batched_over_plain": ms_plain / ms_graph,
+            "batched_over_plain_dense": ms_plain_dense / ms_graph,
+            "autotuned": knobs, "autotune_s": t_tune, "rows_match_single_launches": rows_ok}
 
 
 def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
@@ -254,9 +280,21 @@ def main():
     ap.add_argument("--skip-batched", action="store_true", help="skip the batched 256-file leg")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--legs", default="plain,stats,batched,serving,cpu",
+                    help="comma list of legs to run (plain,stats,batched,serving,cpu); the "
+                         "plain leg gives the JSON line's value")
     ap.add_argument("--calibrate-bytes", type=int, default=0,
                     help="also run the plain copy kernel over this many bytes (counter scaling)")
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
+    if args.no_cpu:
+        legs.discard("cpu")
+    if args.skip_stats:
+        legs.discard("stats")
+    if args.skip_batched:
+        legs.discard("batched")
+    if args.serve_requests <= 0:
+        legs.discard("serving")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -283,7 +321,9 @@ def main():
     qq = ix.query(qsh, qs, qe)
     dc, db_ = ix.alloc(8 * F), ix.alloc(8 * F)
 
-    for _ in range(args.warmup):
+    steps = args.steps if "plain" in legs else 1
+    warm = args.warmup if "plain" in legs else 0
+    for _ in range(warm):
         ix.sweep_async(qq, dc, db_)
     ix.sync()
     log("warmup done")
@@ -292,11 +332,11 @@ def main():
         ix.profile_read(name)
     ix.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         ix.sweep_async(qq, dc, db_)
     ix.sync()
     t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
+    ms_step = (t1 - t0) * 1e3 / steps
     kern = {name: ix.profile_read(name) for name in ("qprep", "partition", "sweep")}
     ix.profile(False)
     counts = dc.to_numpy(np.uint64, F)
@@ -339,15 +379,15 @@ def main():
     if calib:
         rec["calibration_copy"] = calib
 
-    if not args.skip_stats:
+    if "stats" in legs:
         rec["stats"] = stats_leg(ix, qq, cfg, F, args.steps)
-    if not args.skip_batched and args.config == "c2":
+    if "batched" in legs and args.config == "c2":
         rec["batched"] = batched_leg(ix, cfg, F, nshards, D)
-    if args.serve_requests > 0 and args.config == "c2":
+    if "serving" in legs and args.config == "c2":
         rec["serving"] = serving_leg(ix, cfg, F, D, args.serve_requests)
         log(f"serving: {rec['serving']}")
 
-    if not args.no_cpu:
+    if "cpu" in legs:
         rec_cpu, cnt_cpu, sel = cpu_baseline((sh, s, e, fid), (qsh, qs, qe), nshards, F)
         # parity on the same sample: GPU counts must equal the reference path's
         qs_ = ix.query(qsh[sel], qs[sel], qe[sel])

@@ -65,8 +65,8 @@ double autotune(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps, std::str
         }
         set(bv);
     };
-    // method threshold: only matters when the batch has files of both kinds
-    // of size; 1 << 62 makes every file dense
+    // method threshold (files above it go dense): only matters when the
+    // batch has files on both sides of it; 1 << 62 sends every file sparse
     {
         const uint64_t n = std::max<uint64_t>(ix.n, 1);
         std::vector<uint64_t> vals = {n / 16384 + 1, n / 4096 + 1, n / 1024 + 1, n / 256 + 1,
