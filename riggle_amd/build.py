@@ -32,8 +32,6 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 def _compile(src: str) -> str:
     obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
     cmd = [HIPCC, *COMMON, *DEVICE, "-c", src, "-o", obj]
-    if src.endswith(".cpp") and os.path.basename(src) != "capi.cpp" and "cli" in src:
-        pass
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -55,7 +53,7 @@ def build(verbose: bool = True) -> str:
     cli_srcs = sorted(glob.glob(os.path.join(CSRC, "cli_*.cpp")))
     if cli_srcs:
         cli_objs = [_compile(s) for s in cli_srcs]
-        cmd = [HIPCC, "-o", CLI, *cli_objs, "-L" + HERE, "-lchuckle_hip",
+        cmd = [HIPCC, "-o", CLI, *cli_objs, "-L" + HERE, "-lchuckle_hip", "-lz",
                "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -1,0 +1,123 @@
+"""`chuckle add` / `info` on the CPU: the reference's on-disk format
+(meta.json, layer_K.{pos,sid,idx}; src/io/meta.rs, src/io/layer.rs,
+src/io/jump.rs, src/tasks/build.rs:86-276) and BED parsing rules
+(src/io/parse.rs:53-141), checked against the oracle's layer assignment and
+jump tables.  No GPU: `add` without --stats is host work."""
+import gzip
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.cli_cases import CLI, run, scenario
+
+pytestmark = pytest.mark.skipif(not os.path.exists(CLI), reason="chuckle CLI not built")
+
+
+def read_layer(db, shard, k):
+    base = os.path.join(db, "shards", shard, f"layer_{k}")
+    pos = np.fromfile(base + ".pos", np.uint32).reshape(-1, 2)
+    sid = np.fromfile(base + ".sid", np.uint32)
+    idx = np.fromfile(base + ".idx", np.uint64)
+    return pos, sid, idx
+
+
+def check_db(orc, db, expected):
+    """expected: list of (shard name, start, end, sid) of every interval."""
+    meta = json.load(open(os.path.join(db, "meta.json")))
+    assert meta["layer_config"] == {"min_size": 128, "growth_factor": 4}
+    got = []
+    for shard in meta["shards"]:
+        for k in range(meta["num_layers"]):
+            if not os.path.exists(os.path.join(db, "shards", shard, f"layer_{k}.pos")):
+                continue
+            pos, sid, idx = read_layer(db, shard, k)
+            assert len(pos) == len(sid)
+            assert (np.diff(pos[:, 0].astype(np.int64)) >= 0).all(), "layer not sorted by start"
+            for s, e in pos:
+                assert orc.layer_for_size(int(e - s)) == k
+            tile = max(1, orc.layer_max_size(k) // 2)
+            assert idx.tolist() == orc.build_jump_table(pos[:, 0], tile).tolist()
+            got += [(shard, int(s), int(e), int(d)) for (s, e), d in zip(pos, sid)]
+    assert sorted(got) == sorted(expected)
+    return meta
+
+
+def test_add_writes_reference_layers(orc, tmp_path):
+    src, _ = scenario(orc, str(tmp_path), 4, 700, 0, 0, 5_000_000)
+    db = str(tmp_path / "db")
+    run("add", "-i", src, "-d", db)
+    meta = json.load(open(os.path.join(db, "meta.json")))
+    names = {int(k): v["name"] for k, v in meta["sid_map"].items()}
+    assert sorted(names.values()) == [f"source_{i}.bed" for i in range(4)]
+    expected = []
+    for sid, name in names.items():
+        i = int(name.split("_")[1].split(".")[0])
+        c, s, e = orc.generate_bed(100 + i, 700, 5_000_000)
+        expected += [(orc.CHROMS24[a], int(b), int(d), sid) for a, b, d in zip(c, s, e)]
+    check_db(orc, db, expected)
+
+
+def test_add_appends_and_merges(orc, tmp_path):
+    src, _ = scenario(orc, str(tmp_path), 3, 500, 0, 0, 2_000_000)
+    db = str(tmp_path / "db")
+    run("add", "-i", src, "-d", db, "--batch-size", "1")
+    extra = tmp_path / "extra"
+    extra.mkdir()
+    orc.write_bed(str(extra / "late.bed"), *orc.generate_bed(999, 400, 2_000_000))
+    run("add", "-i", str(extra), "-d", db)
+    meta = json.load(open(os.path.join(db, "meta.json")))
+    names = {int(k): v["name"] for k, v in meta["sid_map"].items()}
+    assert names[3] == "late.bed" and len(names) == 4
+    expected = []
+    for sid, name in names.items():
+        seed = 999 if name == "late.bed" else 100 + int(name.split("_")[1].split(".")[0])
+        n = 400 if name == "late.bed" else 500
+        c, s, e = orc.generate_bed(seed, n, 2_000_000)
+        expected += [(orc.CHROMS24[a], int(b), int(d), sid) for a, b, d in zip(c, s, e)]
+    check_db(orc, db, expected)
+
+
+def test_bed_parsing_rules(orc, tmp_path):
+    src = tmp_path / "src"
+    src.mkdir()
+    (src / "a.bed").write_text("# comment\n\nchr1\t10\t20\n30 40\n50\t60\tchr2\tname\t7\n"
+                               "chrX 5 9 extra 1 2\n")
+    with gzip.open(src / "b.bed.gz", "wt") as f:
+        f.write("chr1\t100\t200\n")
+    (src / "ignored.txt").write_text("chr1 1 2\n")
+    db = str(tmp_path / "db")
+    run("add", "-i", str(src), "-d", db)
+    meta = json.load(open(os.path.join(db, "meta.json")))
+    names = {v["name"]: int(k) for k, v in meta["sid_map"].items()}
+    assert set(names) == {"a.bed", "b.bed.gz"}
+    a, b = names["a.bed"], names["b.bed.gz"]
+    check_db(orc, db, [("chr1", 10, 20, a), ("default", 30, 40, a), ("chr2", 50, 60, a),
+                       ("chrX", 5, 9, a), ("chr1", 100, 200, b)])
+
+
+def test_bed_errors_exit_1(tmp_path):
+    src = tmp_path / "src"
+    src.mkdir()
+    (src / "bad.bed").write_text("chr1 10 20\nchr1 30 30\n")
+    r = run("add", "-i", str(src), "-d", str(tmp_path / "db"), check=False)
+    assert r.returncode == 1
+    assert r.stderr.strip() == "Error: Line 2: start (30) must be less than end (30)"
+    (src / "bad.bed").write_text("chr1 10\n")
+    r = run("add", "-i", str(src), "-d", str(tmp_path / "db2"), check=False)
+    assert r.returncode == 1 and "Line 1: invalid format" in r.stderr
+
+
+def test_info(orc, tmp_path):
+    src, _ = scenario(orc, str(tmp_path), 2, 100, 0, 0, 1_000_000)
+    db = str(tmp_path / "db")
+    run("add", "-i", src, "-d", db)
+    out = run("info", "-d", db).stdout.splitlines()
+    assert out[0] == f"Database: {db}"
+    assert out[1] == "Layer config: min_size=128, growth_factor=4"
+    assert out[2].startswith("Layers used: ")
+    assert out[3].startswith("Shards (")
+    assert out[5] == "Sources (2):"
+    assert sorted(out[6:]) == ["  SID    0: source_0.bed", "  SID    1: source_1.bed"] or \
+        sorted(x.split(": ")[1] for x in out[6:]) == ["source_0.bed", "source_1.bed"]
