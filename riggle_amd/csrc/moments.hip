@@ -17,7 +17,8 @@
 //   centring removes its cancellation).
 // The autocorrelation is one complex FFT of the zero-padded H of length
 // N = next_pow2(2n - 1), |X|^2, and the inverse FFT, written as LDS-staged
-// radix-2 passes: one block per signal when N <= FFT_SMALL, otherwise the
+// radix-2 passes: one block per signal when N fits in one block's LDS (8192
+// complex values where the device allows 128 KiB per block), otherwise the
 // four-step split N = N1 * N2 into column FFTs (length N2, twiddled), row
 // FFTs with |X|^2 and the inverse row FFTs fused, and inverse column FFTs.
 // A is FP64; when every value is within 0.25 of an integer and A[0] equals
@@ -418,46 +419,66 @@ __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
     const bool exact = sh_exact != 0;
     Terms carry = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
     int64_t pcarry = 0;
-    // positions 0..n (n+1 values); P[i] = sum_{j<i} G[j]
-    for (uint32_t base = 0; base <= n; base += NT) {
-        const uint32_t i = base + threadIdx.x;
-        const int64_t g = i < n ? (int64_t)G[i] : 0;
-        int64_t gtot;
-        const int64_t gin = block_incl_scan<int64_t>(g, ldsi, gtot);
-        const int64_t P = pcarry + gin - g;  // exclusive
-        pcarry += gtot;
+    // Terms of position i (0..n) given P = P[i] = sum_{j<i} G[j].
+    auto term = [&](uint32_t i, int64_t P) {
         Terms t = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-        if (i <= n) {
-            t.pp = P;
-            const double ph = (double)(P - c * (int64_t)i);
-            const double qm = (double)(PHn - (P - c * (int64_t)i));
-            if (i < n) t.q = dd_from_prod(qm, qm);
-            if (i >= 1 && i < n) {
-                double av = A[i];
-                if (exact) av = rint(av);
-                t.c1 = {av, 0.0};
-                t.c2 = dd_from_prod(av, (double)i);
-                t.e1 = dd_from_prod(ph, ph);
-            }
+        t.pp = P;
+        const double ph = (double)(P - c * (int64_t)i);
+        const double qm = (double)(PHn - (P - c * (int64_t)i));
+        if (i < n) t.q = dd_from_prod(qm, qm);
+        if (i >= 1 && i < n) {
+            double av = A[i];
+            if (exact) av = rint(av);
+            t.c1 = {av, 0.0};
+            t.c2 = dd_from_prod(av, (double)i);
+            t.e1 = dd_from_prod(ph, ph);
+        }
+        return t;
+    };
+    // positions 0..n (n+1 values), EK consecutive positions per thread: a
+    // serial pass over the thread's run, one block scan of the run totals,
+    // then a second serial pass that emits the exclusive prefixes sampled at
+    // the stored block sizes
+    constexpr uint32_t EK = 16;
+    for (uint64_t base = 0; base <= n; base += (uint64_t)NT * EK) {
+        const uint64_t i0 = base + (uint64_t)threadIdx.x * EK;
+        int64_t gsum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < EK; k++)
+            if (i0 + k < n) gsum += (int64_t)G[i0 + k];
+        int64_t gtot;
+        const int64_t gex = block_incl_scan<int64_t>(gsum, ldsi, gtot) - gsum;
+        Terms loc = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        int64_t P = pcarry + gex;
+        for (uint32_t k = 0; k < EK; k++) {
+            const uint64_t i = i0 + k;
+            if (i > n) break;
+            loc = terms_add(loc, term((uint32_t)i, P));
+            if (i < n) P += (int64_t)G[i];
         }
         Terms tot;
-        Terms ex = block_excl_scan_terms(t, lds, tot);
-        ex = terms_add(ex, carry);
-        if (i <= n) {
+        Terms run = terms_add(block_excl_scan_terms(loc, lds, tot), carry);
+        P = pcarry + gex;
+        const size_t rb = (size_t)b * a.nl;
+        for (uint32_t k = 0; k < EK; k++) {
+            const uint64_t i = i0 + k;
+            if (i > n) break;
             const int32_t jl = a.pmapL[i], jm = a.pmapM[i];
-            const size_t rb = (size_t)b * a.nl;
             if (jl >= 0) {
-                a.sPP_L[rb + jl] = ex.pp;
-                a.sC1[rb + jl] = ex.c1;
-                a.sC2[rb + jl] = ex.c2;
-                a.sE1[rb + jl] = ex.e1;
+                a.sPP_L[rb + jl] = run.pp;
+                a.sC1[rb + jl] = run.c1;
+                a.sC2[rb + jl] = run.c2;
+                a.sE1[rb + jl] = run.e1;
             }
             if (jm >= 0) {
-                a.sPP_M[rb + jm] = ex.pp;
-                a.sQ[rb + jm] = ex.q;
+                a.sPP_M[rb + jm] = run.pp;
+                a.sQ[rb + jm] = run.q;
             }
+            run = terms_add(run, term((uint32_t)i, P));
+            if (i < n) P += (int64_t)G[i];
         }
         carry = terms_add(carry, tot);
+        pcarry += gtot;
     }
     // carry now holds the totals over positions 0..n: PP[n+1], Q[n+1] = Q[n]
     __syncthreads();
@@ -569,9 +590,14 @@ void moments_build(Index& ix) {
         CHK_HIP(hipMemcpyAsync(pmapL.p, pl.data(), (n + 2) * 4, hipMemcpyHostToDevice, st));
         CHK_HIP(hipMemcpyAsync(pmapM.p, pm.data(), (n + 2) * 4, hipMemcpyHostToDevice, st));
         CHK_HIP(hipMemcpyAsync(d_ls.p, ls.data(), nl * 4, hipMemcpyHostToDevice, st));
-        // batch of rows bounded by ~4 GiB of FFT workspace
-        const uint32_t B = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(files.size(),
-                                                        (4ull << 30) / (N * sizeof(double2))));
+        // rows per batch: as many as a third of the free HBM holds (FFT
+        // workspace, autocorrelation, coverage), so long chromosomes still
+        // give every CU rows to work on
+        size_t free_b = 0, total_b = 0;
+        CHK_HIP(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t row_bytes = N * sizeof(double2) + (uint64_t)n * 12 + 64 * (uint64_t)nl;
+        const uint32_t B = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>({files.size(), (uint64_t)(free_b / 3) / row_bytes, 4096}));
         const uint32_t ldg = n + 1;
         D.reserve((size_t)B * ldg);
         X.reserve((size_t)B * N);
