@@ -20,7 +20,8 @@
 // following its first element in registers and ranks each 64-element row with
 // two ballots (events below the row's first and last key): rows without an
 // event inside share one rank, the others count the few events inside with
-// cross-lane reads.  Contributions are reduced per DB file in an LDS
+// cross-lane reads.  The row loop reads no LDS, so its atomics are only
+// drained at the tile barrier.  Contributions are reduced per DB file in an LDS
 // histogram that is flushed with one u64 atomic per touched cell when the
 // block moves to another (query file, file group) or ends.
 //
@@ -140,8 +141,10 @@ struct Stage {
     int64_t w[2];
 };
 
+// 8-item tiles fit 96 VGPRs (5 waves per SIMD) without spills; 16-item tiles
+// need more registers than that
 template <int ITEMS>
-__global__ __launch_bounds__(NT) void sweep_kernel(SweepArgs a) {
+__global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs a) {
     constexpr uint32_t IPW = 64u * ITEMS;  // DB elements per wave per tile
     constexpr uint32_t TILE = NT * ITEMS;
     __shared__ unsigned long long hc[FG], hb[FG];
@@ -268,36 +271,54 @@ __global__ __launch_bounds__(NT) void sweep_kernel(SweepArgs a) {
                 wvv = wi <= nq ? lw[wi] : 0;
             };
             window(ufl(db_key(s.x[0])));
+            // key of the wave's last element: when it is below the window's
+            // end, every row of the wave ranks inside the window
+            const uint32_t nvw = nd - wbase < IPW ? nd - wbase : IPW;
+            const uint32_t kl = (nvw - 1u) >> 6, ll = (nvw - 1u) & 63u;
+            uint32_t dkw = 0;
 #pragma unroll
-            for (int k = 0; k < ITEMS; k++) {
-                const uint32_t base = wbase + k * 64u;
-                if (base < nd) {
-                    const uint32_t nv = nd - base < 64u ? nd - base : 64u;
-                    const uint32_t dk = db_key(s.x[k]);
-                    const uint32_t dkf = ufl(dk), dkl = rdl(dk, nv - 1u);
-                    uint32_t ea = popc(__ballot(kv < dkf)), eb = popc(__ballot(kv < dkl));
-                    if (eb == 64u) {  // window exhausted: re-anchor at this row
-                        window(dkf);
-                        ea = popc(__ballot(kv < dkf));
-                        eb = popc(__ballot(kv < dkl));
-                    }
-                    if (ea == eb) {
-                        const uint32_t rsv = rdl(rv, ea);
-                        const int64_t wq = rdl64(wvv, ea);
-                        if (lane < nv) add(m, s.x[k], s.f[k], m.j0 + jlo + ea, rsv, wq);
-                    } else if (eb < 64u) {
+            for (int k = 0; k < ITEMS; k++)
+                if ((uint32_t)k == kl) dkw = rdl(db_key(s.x[k]), ll);
+            if (popc(__ballot(kv < dkw)) < 64u) {
+                // Row rank offset c in [ea, eb] and the prefix values at it,
+                // all from the register window: no LDS read in this loop, so
+                // the rows' atomics are never waited for inside the tile.
+#pragma unroll
+                for (int k = 0; k < ITEMS; k++) {
+                    const uint32_t base = wbase + k * 64u;
+                    if (base < nd) {
+                        const uint32_t nv = nd - base < 64u ? nd - base : 64u;
+                        const uint32_t dk = db_key(s.x[k]);
+                        const uint32_t dkf = ufl(dk), dkl = rdl(dk, nv - 1u);
+                        const uint32_t ea = popc(__ballot(kv < dkf)), eb = popc(__ballot(kv < dkl));
                         uint32_t c = ea;
-                        for (uint32_t t = ea; t < eb; t++) c += rdl(kv, t) < dk ? 1u : 0u;
-                        const uint32_t p = jlo + c;
-                        if (lane < nv) add(m, s.x[k], s.f[k], m.j0 + p, lrs[p], lw[p]);
-                    } else {
-                        // more than 63 events inside one row: per-lane search
-                        uint32_t lo = jlo + ea, hi = nq;
+                        uint32_t rsv = rdl(rv, ea);
+                        int64_t wq = rdl64(wvv, ea);
+                        for (uint32_t t = ea; t < eb; t++) {
+                            const bool past = rdl(kv, t) < dk;
+                            c += past ? 1u : 0u;
+                            const uint32_t r1 = rdl(rv, t + 1);
+                            const int64_t w1 = rdl64(wvv, t + 1);
+                            rsv = past ? r1 : rsv;
+                            wq = past ? w1 : wq;
+                        }
+                        if (lane < nv) add(m, s.x[k], s.f[k], m.j0 + jlo + c, rsv, wq);
+                    }
+                }
+            } else {
+                // more events in the wave's span than the window holds:
+                // per-lane binary search over the staged events
+#pragma unroll
+                for (int k = 0; k < ITEMS; k++) {
+                    const uint32_t i = wbase + k * 64u + lane;
+                    if (i < nd) {
+                        const uint32_t dk = db_key(s.x[k]);
+                        uint32_t lo = 0, hi = nq;
                         while (lo < hi) {
                             const uint32_t mid = (lo + hi) >> 1;
                             if (lkey[mid] < dk) lo = mid + 1; else hi = mid;
                         }
-                        if (lane < nv) add(m, s.x[k], s.f[k], m.j0 + lo, lrs[lo], lw[lo]);
+                        add(m, s.x[k], s.f[k], m.j0 + lo, lrs[lo], lw[lo]);
                     }
                 }
             }
