@@ -210,14 +210,20 @@ def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
     from riggle_amd import gen
     t = time.time()
     g = cfg["genome"]
-    hot = gen.hotspots(g, 0xC40C1E000 + 2)
+    hot = cfg.get("hot") or gen.hotspots(g, 0xC40C1E000 + 2)
     sizes = gen.log_uniform_counts(pool, 1e3, 5e6, seed, stream=0xC5C5)
     files = [gen.intervals(g, int(n), seed, 100 + i, 1000.0, 1.2, hotspot_frac=0.4, hot=hot)
              for i, n in enumerate(sizes)]
     log(f"serving: {pool} distinct query files ({int(sizes.sum())} intervals) in "
         f"{time.time() - t:.1f}s")
     srv = riggle_amd.Server(ix, slots=3, max_batch_files=16)
-    # capacity: closed loop over the pool, twice (the first pass warms up)
+    # each distinct file alone (latency by size), then capacity: closed loop
+    # over the pool, twice (the first pass warms up)
+    alone = []
+    for f in files:
+        r = srv.wait(srv.submit(*f))
+        alone.append((len(f[1]), round(r["latency_ms"], 3)))
+    alone.sort()
     for rep in range(2):
         t = time.perf_counter()
         tk = [srv.submit(*f) for f in files]
@@ -264,7 +270,8 @@ def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
             "capacity_req_per_s": cap, "arrival_rate_per_s": rate, "p50_ms": st["p50_ms"],
             "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
             "files_per_batch": st["files_per_batch"], "swept_per_s": st["swept_per_s"],
-            "offered_swept_per_s": swept / wall, "slots": 3, "max_batch_files": 16}
+            "offered_swept_per_s": swept / wall, "slots": 3, "max_batch_files": 16,
+            "alone_ms_by_size": alone}
 
 
 def main():
@@ -272,7 +279,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c5"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--tune", action="append", default=[],
                     help="launch-shape knob key=value (see chk_tune), may repeat")
@@ -280,12 +287,14 @@ def main():
     ap.add_argument("--skip-batched", action="store_true", help="skip the batched 256-file leg")
     ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests of the serving leg (0 skips it)")
-    ap.add_argument("--legs", default="plain,stats,batched,serving,cpu",
+    ap.add_argument("--legs", default=None,
                     help="comma list of legs to run (plain,stats,batched,serving,cpu); the "
                          "plain leg gives the JSON line's value")
     ap.add_argument("--calibrate-bytes", type=int, default=0,
                     help="also run the plain copy kernel over this many bytes (counter scaling)")
     args = ap.parse_args()
+    if args.legs is None:  # the serving DB is too large for the CPU and moment legs
+        args.legs = "plain,serving" if args.config == "c5" else "plain,stats,batched,serving,cpu"
     legs = set(args.legs.split(","))
     if args.no_cpu:
         legs.discard("cpu")
@@ -383,7 +392,7 @@ def main():
         rec["stats"] = stats_leg(ix, qq, cfg, F, args.steps)
     if "batched" in legs and args.config == "c2":
         rec["batched"] = batched_leg(ix, cfg, F, nshards, D)
-    if "serving" in legs and args.config == "c2":
+    if "serving" in legs and args.config in ("c2", "c5"):
         rec["serving"] = serving_leg(ix, cfg, F, D, args.serve_requests)
         log(f"serving: {rec['serving']}")
 

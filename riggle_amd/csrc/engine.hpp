@@ -83,7 +83,7 @@ struct Moments {
     int stat_rows = 0, stat_exact_rows = 0;  // signals built / whose FFT rounded exactly
 };
 
-// One (shard, layer) segment of the layered start-sorted table with its jump
+// One (shard, file group, layer) segment of the layered start-sorted table with its jump
 // table: jt[t] = first element whose start >= t << shift (src/io/jump.rs:66-113
 // with a power-of-two bucket sized to the segment).
 struct LayerSeg {
@@ -98,8 +98,8 @@ struct LayerSeg {
 // HBM-resident index.  Dense form: per (shard, file group) a start stream
 // (int32 start ascending + u16 group-local file id) and an end stream (int32
 // end ascending + u16 id), 12 bytes per interval.  Sparse form: per (shard,
-// size layer) the intervals sorted by start as int32 start / int32 end / u32
-// file id plus a jump table, 12 bytes per interval + the tables.
+// file group, size layer) the intervals sorted by start as int32 start /
+// int32 end / u32 file id plus a jump table, 12 bytes per interval + tables.
 struct Index {
     Tuning tune;
     uint32_t tune_version = 0;
@@ -117,7 +117,7 @@ struct Index {
     uint32_t nlayers = 0;
     uint64_t span = 0;                // sum over shards of the largest start
     std::vector<uint64_t> layer_n;    // intervals per size layer (all shards)
-    std::vector<LayerSeg> lseg;       // [nshards * nlayers]
+    std::vector<LayerSeg> lseg;       // [(nshards * ngroups + group) * nlayers + layer]
     DevBuf<LayerSeg> d_lseg;
     DevBuf<int32_t> l_start, l_end;
     DevBuf<uint32_t> l_fid;
@@ -153,9 +153,10 @@ struct PlaceFile {
     uint32_t n, row, drank, pad;  // drank = rank among the dense files, or ~0
 };
 
-// Sparse work item: query intervals [begin, end) (upload order) of one file.
+// Sparse work item: query intervals [begin, end) (placed order) of one file
+// against the DB files of one group.
 struct SparseItem {
-    uint32_t row, begin, end, pad;
+    uint32_t row, begin, end, group;
 };
 
 struct Query {
@@ -165,6 +166,7 @@ struct Query {
     uint64_t n_in = 0;                // intervals given
     uint64_t n = 0;                   // intervals kept (on shards of the DB)
     std::vector<double> file_len;     // total length per query file
+    std::vector<double> file_visits;  // estimated sparse-path visits per query file
     std::vector<uint64_t> fs_n;       // [nfiles * nshards] intervals per (file, shard)
     std::vector<uint64_t> file_off;   // canonical layout: first interval of each file
     PinnedBuf<uint32_t> st_shard, st_start, st_end;  // canonical layout, host staging
@@ -226,8 +228,11 @@ void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const 
                   const uint32_t* qfile);
 // Canonical layout (grouped by file) in the staging buffers -> device, then plan.
 void query_stage_commit(Query& q);
-// Whether a query file of n intervals and total length takes the dense path.
-bool file_is_dense(const Index& ix, uint64_t n, double total_len);
+// Whether a query file of n intervals and estimated sparse visits takes the
+// dense path.
+bool file_is_dense(const Index& ix, uint64_t n, double visits);
+// Estimated sparse-path visits per query file (device, from the jump tables).
+void query_visits(Query& q);
 // Method per file and device placement for the current tuning.
 void query_place(Query& q);
 // Placement plus the host work plan of the dense and sparse paths for the

@@ -7,8 +7,8 @@
 // reference's size classes nor its look-back windows (src/io/meta.rs:33-81,
 // src/io/jump.rs:66-113, src/sweep/query.rs:84-117).
 //
-// Sparse form: per (shard, size layer) the intervals sorted by start with
-// their ends and file ids, plus a jump table -- the reference's layer files
+// Sparse form: per (shard, file group, size layer) the intervals sorted by
+// start with their ends and file ids, plus a jump table -- the reference's layer files
 // (src/io/layer.rs:34-128, src/tasks/build.rs:196-247) held in HBM, used to
 // enumerate the overlapping pairs of small query files directly.
 //
@@ -27,6 +27,8 @@ namespace chk {
 namespace {
 
 constexpr int NT = 256;
+constexpr uint32_t SPARSE_UNIT = 32;  // fixed visit charge per (interval, group, layer) lookup
+constexpr double SPARSE_WEIGHT = 4.0; // sparse visit cost in dense-element units (measured)
 
 inline unsigned nblk(size_t n) { return (unsigned)((n + NT - 1) / NT); }
 
@@ -56,12 +58,14 @@ __host__ __device__ inline uint32_t layer_of(uint32_t len) {
     return (l2 - 7u) / 2u + 1u;
 }
 
-// key = ((shard * nlayers + layer) << 31) | start; payload = interval index
+// key = (((shard * ngroups + group) * nlayers + layer) << 31) | start;
+// payload = interval index
 __global__ void make_layer_keys(const uint32_t* shard, const uint32_t* start, const uint32_t* end,
-                                size_t n, uint32_t nlayers, uint64_t* keys, uint32_t* vals) {
+                                const uint32_t* fid, size_t n, uint32_t ngroups, uint32_t nlayers,
+                                uint64_t* keys, uint32_t* vals) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint64_t seg = (uint64_t)shard[i] * nlayers + layer_of(end[i] - start[i]);
+    const uint64_t seg = ((uint64_t)shard[i] * ngroups + fid[i] / FG) * nlayers + layer_of(end[i] - start[i]);
     keys[i] = (seg << 31) | (uint64_t)start[i];
     vals[i] = (uint32_t)i;
 }
@@ -175,26 +179,28 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
     ix.l_fid.reserve(n);
     hipStream_t st = ix.stream;
     // layer segment sizes on the host (for the jump-table geometry)
-    std::vector<uint64_t> lcount((size_t)S * NL, 0);
-    std::vector<uint32_t> lmax((size_t)S * NL, 0);
+    // layered segments per (shard, file group, layer)
+    const size_t NSEG = (size_t)S * NG * NL;
+    std::vector<uint64_t> lcount(NSEG, 0);
+    std::vector<uint32_t> lmax(NSEG, 0);
     for (uint64_t i = 0; i < n; i++) {
-        const size_t b = (size_t)shard[i] * NL + layer_of(end[i] - start[i]);
+        const size_t b = ((size_t)shard[i] * NG + fid[i] / FG) * NL + layer_of(end[i] - start[i]);
         lcount[b]++;
         lmax[b] = std::max(lmax[b], start[i]);
     }
-    ix.lseg.assign((size_t)S * NL, LayerSeg{});
+    ix.lseg.assign(NSEG, LayerSeg{});
     ix.layer_n.assign(NL, 0);
     ix.span = 0;
     for (uint32_t sh = 0; sh < S; sh++) {
         uint32_t mx = 0;
-        for (uint32_t k = 0; k < NL; k++) {
-            ix.layer_n[k] += lcount[(size_t)sh * NL + k];
-            mx = std::max(mx, lmax[(size_t)sh * NL + k]);
+        for (size_t b = (size_t)sh * NG * NL; b < (size_t)(sh + 1) * NG * NL; b++) {
+            ix.layer_n[b % NL] += lcount[b];
+            mx = std::max(mx, lmax[b]);
         }
         ix.span += mx;
     }
     uint64_t off = 0, jto = 0;
-    for (size_t b = 0; b < (size_t)S * NL; b++) {
+    for (size_t b = 0; b < NSEG; b++) {
         LayerSeg& L = ix.lseg[b];
         const uint32_t layer = (uint32_t)(b % NL);
         L.off = off;
@@ -247,9 +253,9 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
         CHK_HIP(hipGetLastError());
     }
     // layered table + jump tables
-    const uint32_t lbits = 31 + bits_for((uint64_t)S * NL - 1);
+    const uint32_t lbits = 31 + bits_for((uint64_t)NSEG - 1);
     hipLaunchKernelGGL(make_layer_keys, dim3(nblk(n)), dim3(NT), 0, st, d_shard.p, d_start.p,
-                       d_end.p, (size_t)n, NL, keys.p, vals.p);
+                       d_end.p, d_fid.p, (size_t)n, NG, NL, keys.p, vals.p);
     radix_sort_u64(keys.p, vals.p, n, lbits, ix.scratch, st);
     hipLaunchKernelGGL(gather_layer, dim3(nblk(n)), dim3(NT), 0, st, keys.p, vals.p, (size_t)n,
                        d_end.p, d_fid.p, ix.l_start.p, ix.l_end.p, ix.l_fid.p);
@@ -324,6 +330,7 @@ void query_stage_commit(Query& q) {
     q.staged = true;
     q.placed = false;
     q.plan_version = 0xFFFFFFFFu;
+    query_visits(q);
     query_plan(q);
 }
 
@@ -353,41 +360,106 @@ __global__ void place_kernel(const PlaceFile* files, uint32_t nfiles, uint64_t n
     }
 }
 
+// Estimated work of the sparse path per query file: for every (interval,
+// layer) the DB starts in [q.s - lookback + 1, q.e) counted from the jump
+// table at bucket granularity, plus a fixed charge per lookup.  The canonical
+// layout groups intervals by file (file_off).
+__global__ void sparse_visits(const uint32_t* shard, const uint32_t* start, const uint32_t* end,
+                              uint64_t n, const uint64_t* file_off, uint32_t nfiles,
+                              const LayerSeg* lseg, uint32_t nsub, const uint32_t* jt,
+                              unsigned long long* visits) {
+    // each thread sums a contiguous run of VR intervals (one file or two),
+    // then a wave whose lanes all ended in the same file adds one total
+    constexpr uint32_t VR = 32;
+    const uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * VR;
+    uint32_t f = 0xFFFFFFFFu;
+    unsigned long long v = 0;
+    if (i0 < n) {
+        uint32_t lo = 0, hi = nfiles;  // file of interval i0: last with file_off <= i0
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (file_off[mid] <= i0) lo = mid; else hi = mid;
+        }
+        f = lo;
+        const uint64_t i1 = i0 + VR < n ? i0 + VR : n;
+        for (uint64_t i = i0; i < i1; i++) {
+            while (i >= file_off[f + 1]) {  // next non-empty file
+                if (v) atomicAdd(&visits[f], v);
+                v = 0;
+                f++;
+            }
+            const uint32_t s = shard[i];
+            const int64_t qs = start[i], qe = end[i];
+            for (uint32_t K = 0; K < nsub; K++) {  // every (file group, layer) of the shard
+                const LayerSeg L = lseg[(size_t)s * nsub + K];
+                if (L.n == 0) continue;
+                const int64_t a = qs - (int64_t)L.lookback + 1;
+                const uint32_t b0 = (uint32_t)(a > 0 ? a : 0) >> L.shift;
+                const uint32_t b1 = ((uint32_t)qe >> L.shift) + 1;
+                const uint32_t j0 = b0 < L.nb ? jt[L.jt + b0] : L.n;
+                const uint32_t j1 = b1 < L.nb ? jt[L.jt + b1] : L.n;
+                v += (j1 > j0 ? j1 - j0 : 0) + SPARSE_UNIT;
+            }
+        }
+    }
+    const uint32_t f0 = __builtin_amdgcn_readfirstlane(f);
+    if (__ballot(f == f0) == __ballot(1)) {
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        if ((threadIdx.x & 63) == 0 && f0 != 0xFFFFFFFFu && v) atomicAdd(&visits[f0], v);
+    } else if (f != 0xFFFFFFFFu && v) {
+        atomicAdd(&visits[f], v);
+    }
+}
+
 }  // namespace
 
 // Method per file and device placement.  The dense path streams the whole DB
 // (2 |D| elements whatever the file's size); the sparse path visits, per
 // query interval and size layer K, the DB starts within lookback_K + its
-// length before its end -- about N_K (lookback_K + len) / span of them.  A
-// file goes sparse when SPARSE_COST times its expected visits is below the
-// dense pass (SPARSE_COST: the sparse path's cost per visited element against
-// the dense path's, with headroom for DB hotspots the query shares), or, with
-// an explicit sweep.sparse_max, when it has at most that many intervals.
+// length before its end, counted per file on the device from the jump tables
+// (sparse_visits) when the batch is staged.  A file goes sparse when its
+// weighted visits are below the
+// dense pass (the sparse path's cost per visited element against
+// the dense path's), or, with an explicit sweep.sparse_max, when it has at
+// most that many intervals.
 // Files are ordered dense first, so each sparse file is a contiguous range
 // and the dense files' events are a prefix.  Redone only when the choice
 // changes.
-constexpr double SPARSE_COST = 8.0;
-
-bool file_is_dense(const Index& ix, uint64_t n, double total_len) {
+bool file_is_dense(const Index& ix, uint64_t n, double visits) {
     if (!n) return false;
     if (ix.tune.method == 1) return true;
     if (ix.tune.method == 2) return false;
     if (ix.tune.sparse_max) return n > ix.tune.sparse_max;
-    double lb_sum = 0.0, n_sum = 0.0;  // sum_K N_K lookback_K, sum_K N_K
-    for (uint32_t k = 0; k < ix.nlayers; k++) {
-        lb_sum += (double)ix.layer_n[k] * (double)ix.lseg[k].lookback;
-        n_sum += (double)ix.layer_n[k];
-    }
-    const double span = (double)std::max<uint64_t>(ix.span, 1);
-    const double visits = ((double)n * lb_sum + total_len * n_sum) / span;
-    return SPARSE_COST * visits > 2.0 * (double)ix.n + 2.0 * (double)n;
+    // measured cost of a sparse visit in dense-element units
+    return SPARSE_WEIGHT * visits > 2.0 * (double)ix.n + 2.0 * (double)n;
+}
+
+void query_visits(Query& q) {
+    Index& ix = *q.idx;
+    const uint32_t R = q.nfiles;
+    q.file_visits.assign(R, 0.0);
+    if (!q.n || ix.n == 0) return;
+    DevBuf<uint64_t> d_off, d_vis;
+    d_off.reserve(R + 1);
+    d_vis.reserve(R);
+    CHK_HIP(hipMemcpyAsync(d_off.p, q.file_off.data(), (R + 1) * 8, hipMemcpyHostToDevice, q.stream));
+    zero_u64(d_vis.p, R, q.stream);
+    const unsigned g = (unsigned)((q.n + (uint64_t)NT * 32 - 1) / ((uint64_t)NT * 32));
+    hipLaunchKernelGGL(sparse_visits, dim3(g), dim3(NT), 0, q.stream, q.c_shard.p, q.c_start.p,
+                       q.c_end.p, q.n, d_off.p, R, ix.d_lseg.p, ix.ngroups * ix.nlayers, ix.jt.p,
+                       reinterpret_cast<unsigned long long*>(d_vis.p));
+    CHK_HIP(hipGetLastError());
+    std::vector<uint64_t> h(R);
+    CHK_HIP(hipMemcpyAsync(h.data(), d_vis.p, R * 8, hipMemcpyDeviceToHost, q.stream));
+    CHK_HIP(hipStreamSynchronize(q.stream));
+    for (uint32_t r = 0; r < R; r++) q.file_visits[r] = (double)h[r];
 }
 
 void query_place(Query& q) {
     Index& ix = *q.idx;
     const uint32_t S = ix.nshards, R = q.nfiles;
     std::vector<uint8_t> dense(R, 0);
-    for (uint32_t r = 0; r < R; r++) dense[r] = file_is_dense(ix, q.file_n[r], q.file_len[r]);
+    for (uint32_t r = 0; r < R; r++) dense[r] = file_is_dense(ix, q.file_n[r], q.file_visits[r]);
     if (q.placed && dense == q.file_dense) return;
     q.file_dense = dense;
     q.dense_rows.clear();

@@ -612,9 +612,9 @@ void moments_build(Index& ix) {
         sC2.reserve((size_t)B * nl);
         sE1.reserve((size_t)B * nl);
         sQ.reserve((size_t)B * nl);
-        const uint32_t NL = ix.nlayers;
-        const LayerSeg& Ll = ix.lseg[(size_t)s * NL + NL - 1];
-        const uint64_t soff = ix.lseg[(size_t)s * NL].off, scnt = Ll.off + Ll.n - soff;
+        const size_t NSUB = (size_t)ix.ngroups * ix.nlayers;  // segments of one shard
+        const LayerSeg& Ll = ix.lseg[(size_t)s * NSUB + NSUB - 1];
+        const uint64_t soff = ix.lseg[(size_t)s * NSUB].off, scnt = Ll.off + Ll.n - soff;
         for (size_t f0 = 0; f0 < files.size(); f0 += B) {
             const uint32_t nb = (uint32_t)std::min<size_t>(B, files.size() - f0);
             std::vector<int64_t> oo(nb);

@@ -26,10 +26,11 @@
 // block moves to another (query file, file group) or ends.
 //
 // Sparse path.  A small query file is swept the reference's way
-// (src/sweep/query.rs:84-117): for each query interval and each size layer,
-// a jump-table lookup finds the first DB start that can overlap it and a
-// wave scans the layer's starts up to q.end, adding every overlapping pair.
-// Its cost follows the query's pairs instead of the DB's size.
+// (src/sweep/query.rs:84-117): for each query interval and each size layer
+// (of each DB file group), a jump-table lookup finds the first DB start that
+// can overlap it and a wave scans the layer's starts up to q.end, adding
+// every overlapping pair.  Its cost follows the query's pairs instead of the
+// DB's size.
 #include <algorithm>
 
 #include "engine.hpp"
@@ -411,7 +412,7 @@ struct SparseArgs {
     const uint32_t* start;
     const uint32_t* end;
     const LayerSeg* lseg;
-    uint32_t nlayers, F;
+    uint32_t ngroups, nlayers, F;
     const int32_t* l_start;
     const int32_t* l_end;
     const uint32_t* l_fid;
@@ -420,33 +421,30 @@ struct SparseArgs {
     unsigned long long* bp;
 };
 
-// One block per work item.  The item's (query interval, size layer) pairs
-// are dealt to the block's waves; per pair a wave looks up the first DB start
-// that can reach the interval (start >= q.s - lookback + 1) in the layer's
-// jump table and scans the layer's starts up to q.e 64 at a time, loading the
-// next 64 starts while the current ones are tested.  With LH the pairs are
-// reduced in an LDS histogram of all F files (flushed once per item), else
-// added to the output directly.
-template <bool LH>
+// One block per work item (query intervals of one file x one DB file group).
+// The item's (query interval, size layer) pairs are dealt to the block's
+// waves; per pair a wave looks up the first DB start of the group's layer
+// that can reach the interval (start >= q.s - lookback + 1) in the jump table
+// and scans the layer's starts up to q.e 64 at a time, loading the next 64
+// starts while the current ones are tested.  Pairs are reduced in an LDS
+// histogram of the group's files, flushed once per item.
 __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
-    extern __shared__ unsigned long long sh[];
-    unsigned long long* hc = sh;
-    unsigned long long* hb = sh + a.F;
+    __shared__ unsigned long long hc[FG], hb[FG];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wv = ufl(tid >> 6);
     const SparseItem it = a.items[blockIdx.x];
-    const uint64_t rowcell = (uint64_t)it.row * a.F;
+    const uint32_t g = it.group;
+    const uint32_t fbase = g * FG;
+    const uint32_t nf = min(FG, a.F - fbase);
     const uint32_t NL = a.nlayers;
-    if (LH) {
-        for (uint32_t f = tid; f < a.F; f += NT) hc[f] = hb[f] = 0ull;
-        __syncthreads();
-    }
+    for (uint32_t f = tid; f < nf; f += NT) hc[f] = hb[f] = 0ull;
+    __syncthreads();
     const uint32_t nunits = (it.end - it.begin) * NL;
     for (uint32_t u = wv; u < nunits; u += NT / 64) {
         const uint32_t qi = u / NL, K = u - qi * NL;
         const uint32_t i = it.begin + qi;
         const uint32_t s = ufl(a.shard[i]);
-        const LayerSeg L = a.lseg[(size_t)s * NL + K];
+        const LayerSeg L = a.lseg[((size_t)s * a.ngroups + g) * NL + K];
         if (L.n == 0) continue;
         const int32_t qs = (int32_t)ufl(a.start[i]), qe = (int32_t)ufl(a.end[i]);
         const int64_t lo64 = (int64_t)qs - (int64_t)L.lookback + 1;
@@ -466,15 +464,10 @@ __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
                 const uint32_t idx = p + lane;
                 const int32_t en = a.l_end[L.off + idx];
                 if (en > qs) {
-                    const uint32_t f = a.l_fid[L.off + idx];
+                    const uint32_t f = a.l_fid[L.off + idx] - fbase;
                     const int64_t ov = (int64_t)min(qe, en) - (int64_t)max(qs, st);
-                    if (LH) {
-                        atomicAdd(&hc[f], 1ull);
-                        atomicAdd(&hb[f], (unsigned long long)ov);
-                    } else {
-                        atomicAdd(&a.counts[rowcell + f], 1ull);
-                        atomicAdd(&a.bp[rowcell + f], (unsigned long long)ov);
-                    }
+                    atomicAdd(&hc[f], 1ull);
+                    atomicAdd(&hb[f], (unsigned long long)ov);
                 }
             }
             if (!more || pn >= L.n) break;
@@ -482,19 +475,16 @@ __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
             st = stn;
         }
     }
-    if (LH) {
-        __syncthreads();
-        for (uint32_t f = tid; f < a.F; f += NT) {
-            const unsigned long long cv = hc[f], bv = hb[f];
-            if (cv | bv) {
-                atomicAdd(&a.counts[rowcell + f], cv);
-                atomicAdd(&a.bp[rowcell + f], bv);
-            }
+    __syncthreads();
+    const uint64_t cell = (uint64_t)it.row * a.F + fbase;
+    for (uint32_t f = tid; f < nf; f += NT) {
+        const unsigned long long cv = hc[f], bv = hb[f];
+        if (cv | bv) {
+            atomicAdd(&a.counts[cell + f], cv);
+            atomicAdd(&a.bp[cell + f], bv);
         }
     }
 }
-
-constexpr uint32_t SPARSE_LDS_MAX_F = 2048;  // LDS histogram of all files up to this many
 
 template <int ITEMS>
 void launch_sweep(const SweepArgs& args, uint32_t grid, hipStream_t st) {
@@ -579,9 +569,9 @@ void query_plan(Query& q) {
         }
         CJ[q.nchunks] = J.empty() ? 0 : (uint32_t)J.size() - 1;
     }
-    // sparse items: enough of them to fill the device even for one small
-    // file, at most SPI intervals each
-    const uint64_t n_sparse = q.n - q.n_dense;
+    // sparse items (file x interval chunk x DB file group): enough of them to
+    // fill the device even for one small file, at most SPI intervals each
+    const uint64_t n_sparse = (q.n - q.n_dense) * NG;
     const uint32_t spi =
         tu.sparse_items ? tu.sparse_items
                         : (uint32_t)std::max<uint64_t>(4, std::min<uint64_t>(SPI, (n_sparse + ix.ncu * 16 - 1) / ((uint64_t)ix.ncu * 16)));
@@ -591,13 +581,15 @@ void query_plan(Query& q) {
         uint64_t p = q.n_dense;
         for (uint32_t r = 0; r < q.nfiles; r++) {
             if (q.file_dense[r] || !q.file_n[r]) continue;
-            for (uint64_t b = 0; b < q.file_n[r]; b += spi) {
-                SparseItem it{};
-                it.row = r;
-                it.begin = (uint32_t)(p + b);
-                it.end = (uint32_t)(p + std::min<uint64_t>(q.file_n[r], b + spi));
-                I.push_back(it);
-            }
+            for (uint32_t g = 0; g < NG; g++)
+                for (uint64_t b = 0; b < q.file_n[r]; b += spi) {
+                    SparseItem it{};
+                    it.row = r;
+                    it.begin = (uint32_t)(p + b);
+                    it.end = (uint32_t)(p + std::min<uint64_t>(q.file_n[r], b + spi));
+                    it.group = g;
+                    I.push_back(it);
+                }
             p += q.file_n[r];
         }
     }
@@ -651,16 +643,13 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
         ix.prof.end("sweep", t0, st);
     }
     if (!q.items_host.empty()) {
-        SparseArgs sa{q.sitems.p, q.shard.p, q.start.p, q.end.p, ix.d_lseg.p, ix.nlayers, F,
+        SparseArgs sa{q.sitems.p, q.shard.p, q.start.p, q.end.p, ix.d_lseg.p, ix.ngroups, ix.nlayers, F,
                       ix.l_start.p, ix.l_end.p, ix.l_fid.p, ix.jt.p,
                       reinterpret_cast<unsigned long long*>(d_counts),
                       reinterpret_cast<unsigned long long*>(d_bp)};
         const uint32_t ni = (uint32_t)q.items_host.size();
         hipEvent_t t0 = ix.prof.begin(st);
-        if (F <= SPARSE_LDS_MAX_F)
-            hipLaunchKernelGGL(sparse_kernel<true>, dim3(ni), dim3(NT), 16 * F, st, sa);
-        else
-            hipLaunchKernelGGL(sparse_kernel<false>, dim3(ni), dim3(NT), 0, st, sa);
+        hipLaunchKernelGGL(sparse_kernel, dim3(ni), dim3(NT), 0, st, sa);
         CHK_HIP(hipGetLastError());
         ix.prof.end("sparse", t0, st);
     }

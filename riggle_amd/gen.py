@@ -92,6 +92,33 @@ def make_db(genome_sizes, counts, seed, hotspot_frac, median, sigma, hot=None):
     return shard, start, end, fid
 
 
+def make_db_chunked(genome_sizes, counts, seed, hotspot_frac, median, sigma, hot=None,
+                    chunk=1 << 25, threads=8):
+    """make_db for billions of intervals: chunk c is drawn from stream
+    (1 << 20) + c, chunks in parallel threads, into preallocated uint32
+    arrays (shard, start, end, fid)."""
+    import concurrent.futures as cf
+    counts = np.asarray(counts, np.int64)
+    N = int(counts.sum())
+    shard = np.empty(N, np.uint32)
+    start = np.empty(N, np.uint32)
+    end = np.empty(N, np.uint32)
+
+    def work(c):
+        a = c * chunk
+        n = min(chunk, N - a)
+        sh, s, e = intervals(genome_sizes, n, seed, (1 << 20) + c, median, sigma,
+                             hotspot_frac=hotspot_frac, hot=hot)
+        shard[a:a + n] = sh
+        start[a:a + n] = s
+        end[a:a + n] = e
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(work, range((N + chunk - 1) // chunk)))
+    fid = np.repeat(np.arange(len(counts), dtype=np.uint32), counts)
+    return shard, start, end, fid
+
+
 def config(name: str, seed: int = 0xC40C1E000):
     """Returns a dict with genome, db arrays, query arrays for a benchmark config."""
     if name == "c1":
@@ -117,4 +144,15 @@ def config(name: str, seed: int = 0xC40C1E000):
         sh, s, e = intervals(g, int(qc.sum()), seed + 4, 2, 1000.0, 1.2, hotspot_frac=0.3, hot=hot)
         qf = np.repeat(np.arange(256, dtype=np.uint32), qc)
         return {"genome": g, "db": db, "nfiles": 1000, "query": (sh, s, e, qf), "nq": 256}
+    if name == "c5":
+        # serving: 10,000 files log-uniform 10k-1M (about 2.15e9 intervals),
+        # 40 % hotspot; one sample query of 1M intervals (the serving leg
+        # draws its own request stream)
+        g = genome("c2")
+        hot = hotspots(g, seed + 5)
+        counts = log_uniform_counts(10_000, 1e4, 1e6, seed + 5)
+        db = make_db_chunked(g, counts, seed + 5, 0.4, 1000.0, 1.2, hot)
+        q = intervals(g, 1_000_000, seed + 5, 2, 1000.0, 1.2, hotspot_frac=0.4, hot=hot)
+        return {"genome": g, "db": db, "nfiles": 10_000, "hot": hot,
+                "query": q + (np.zeros(1_000_000, np.uint32),), "nq": 1}
     raise ValueError(name)
