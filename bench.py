@@ -116,6 +116,9 @@ def stats_leg(ix, qq, cfg, F, steps):
     ix.profile(False)
     flags = dfl.to_numpy(np.uint8, cells)
     emitted = int((flags & 1).sum())
+    stats_leg.outputs = {"q": q2, "p": dp.to_numpy(np.float64, cells),
+                         "llr": dl.to_numpy(np.float64, cells), "flags": flags,
+                         "bp": db_.to_numpy(np.uint64, cells)}
     pv_ms = ms_pv / max(n_pv, 1)
     return {"moment_signals": rows, "moment_signals_exact": exact, "moment_bins": bins,
             "moments_build_s": t_mom, "moments_bins_per_s": bins / t_mom,
@@ -124,6 +127,86 @@ def stats_leg(ix, qq, cfg, F, steps):
             "pvalues_per_s_kernel": emitted / (pv_ms * 1e-3),
             "sweep_plus_pvalues_ms_per_step": ms_step,
             "pvalues_per_s_end_to_end": emitted / (ms_step * 1e-3)}
+
+
+C3_WIDTHS = [1, 2, 3, 4, 6, 10, 16, 25, 40, 63, 100, 158, 251, 398, 631, 1000]
+
+
+def cpu_moments_and_precision(ix, cfg, F, stats_out):
+    """Checks of the moment / p-value path against the oracle (FP64, exact
+    integer moments) and the oracle's reference-method moment time.
+    Moments: exact tables of sampled (file, chromosome) tracks vs the device
+    tables, at all stored block sizes and at the 16 widths 1..1000 bins.
+    p-values: every emitted cell of the query vs the oracle's FP64 evaluation
+    of compute_analytic_stats on the device tables."""
+    import oracle
+    sh, s, e, fid = cfg["db"]
+    genome = cfg["genome"]
+    S = len(genome)
+    so, sl, gmean, gvar = ix.download_moments()
+    sample = [(f, c) for c in (S - 4, S - 3) for f in range(0, F, max(1, F // 8))][:16]
+    worst = {"mean_all": 0.0, "var_all": 0.0, "mean_c3": 0.0, "var_c3": 0.0}
+    t_ref, bins_ref = 0.0, 0
+    order = np.lexsort((sh, fid))
+    fs, ss = fid[order], sh[order]
+    for f, c in sample:
+        lo = np.searchsorted(fs, f, "left")
+        hi = np.searchsorted(fs, f, "right")
+        sel = order[lo:hi][ss[lo:hi] == c]
+        if len(sel) == 0 or so[f * S + c] < 0:
+            continue
+        nb = (genome[c][1] + 99) // 100
+        ls, xm, xv = oracle.chrom_moments_exact(s[sel], e[sel], nb)
+        o, n = so[f * S + c], sl[f * S + c]
+        gm, gv = gmean[o:o + n], gvar[o:o + n]
+        rel = lambda a, b: float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+        worst["mean_all"] = max(worst["mean_all"], rel(gm, xm))
+        worst["var_all"] = max(worst["var_all"], rel(gv, xv))
+        idx = [oracle.compact_index(L) for L in C3_WIDTHS if L < nb]
+        worst["mean_c3"] = max(worst["mean_c3"], rel(gm[idx], xm[idx]))
+        worst["var_c3"] = max(worst["var_c3"], rel(gv[idx], xv[idx]))
+        # the reference method on the CPU (f32 depth signal, FP64 FFT moments)
+        t = time.perf_counter()
+        g = oracle.depth_signal_f32(s[sel], e[sel], nb)
+        oracle.moments_reference(g, ls)
+        t_ref += time.perf_counter() - t
+        bins_ref += nb
+    # p-values of the query: oracle FP64 evaluation on the device tables
+    qsh, qs, qe, _ = cfg["query"]
+    groups = {}
+    for c in range(S):
+        m = qsh == c
+        if m.any():
+            groups[c] = oracle.group_lengths(qe[m] - qs[m])
+    dmax, checked = 0.0, 0
+    for f in range(F):
+        if not stats_out["flags"][f] & 1:
+            continue
+        tabs, parts = [], [[], [], [], [], []]
+        for c, g4 in groups.items():
+            o = so[f * S + c]
+            slot = -1
+            if o >= 0:
+                slot = len(tabs)
+                tabs.append(((genome[c][1] + 99) // 100, gmean[o:o + sl[f * S + c]],
+                             gvar[o:o + sl[f * S + c]]))
+            for a, v in zip(parts, g4 + (np.full(len(g4[0]), slot, np.int32),)):
+                a.append(v)
+        res = oracle.analytic_stats(tuple(np.concatenate(a) for a in parts), None, tabs,
+                                    float(stats_out["bp"][f]) / 100.0)
+        if res is None:
+            continue
+        ml = lambda p: np.inf if p <= 0 else -np.log10(p)
+        a, b = ml(stats_out["p"][f]), ml(res[0])
+        if np.isfinite(a) and np.isfinite(b):
+            dmax = max(dmax, abs(a - b))
+        checked += 1
+    return {"moments_rel_err_max": worst, "moment_signals_checked": len(sample),
+            "c3_widths_bins": C3_WIDTHS, "pvalue_cells_checked": checked,
+            "pvalue_abs_err_mlog10p_max": dmax,
+            "moments_reference_method_cpu": {"bins_per_s": bins_ref / max(t_ref, 1e-9),
+                                             "cores": 1, "kind": "port",
+                                             "sample": f"{len(sample)} (file, chromosome) tracks"}}
 
 
 def batched_leg(ix, cfg, F, nshards, D):
@@ -402,6 +485,11 @@ def main():
         qs_ = ix.query(qsh[sel], qs[sel], qe[sel])
         gc, _ = ix.sweep(qs_)
         rec_cpu["sample_counts_bit_exact"] = bool((gc[0] == cnt_cpu[0].astype(np.uint64)).all())
+        if "stats" in rec:
+            t = time.time()
+            rec_cpu["moments_and_pvalues"] = cpu_moments_and_precision(ix, cfg, F, stats_leg.outputs)
+            log(f"cpu: moment / p-value checks in {time.time() - t:.1f}s: "
+                f"{rec_cpu['moments_and_pvalues']['moments_rel_err_max']}")
         rec["cpu_baseline"] = rec_cpu
         rec["speedup_vs_cpu"] = swept / rec_cpu["value"]
         log(f"cpu baseline {rec_cpu['value']:.3e} intervals/s; sample parity "

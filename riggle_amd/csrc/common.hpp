@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -38,6 +39,10 @@ void set_last_error(const std::string& msg);
     } while (0)
 
 // Owning device allocation (hipMalloc'd); grows on demand, never shrinks.
+// The first allocation is exact; a buffer that has to grow again grows to at
+// least 1.5x its capacity, so buffers reused across batches of varying size
+// (query and serving buffers) reallocate -- a device-synchronising free --
+// only a few times.
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -53,6 +58,7 @@ struct DevBuf {
     }
     void reserve(size_t count) {
         if (count <= n) return;
+        if (n) count = std::max(count, n + n / 2);
         release();
         size_t bytes = (count ? count : 1) * sizeof(T);
         hipError_t e = hipMalloc(&p, bytes);
@@ -77,6 +83,7 @@ struct PinnedBuf {
     PinnedBuf& operator=(const PinnedBuf&) = delete;
     void reserve(size_t count) {
         if (count <= n) return;
+        if (n) count = std::max(count, n + n / 2);
         if (p) (void)hipHostFree(p);
         p = nullptr;
         n = 0;
