@@ -58,9 +58,9 @@ __global__ __launch_bounds__(NT) void scan_tile_sums(const T* data, size_t n, T*
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// Single block: exclusive scan of sums[0..nb) in place, total to *out_total.
+// Single block: exclusive scan of sums[0..nb) in place (scans of many tiles).
 template <typename T>
-__global__ __launch_bounds__(NT) void scan_sums_single(T* sums, size_t nb, T* out_total) {
+__global__ __launch_bounds__(NT) void scan_sums_single(T* sums, size_t nb) {
     __shared__ T lds[NT / 64 + 1];
     T carry = 0;
     for (size_t c0 = 0; c0 < nb; c0 += SC_TILE) {
@@ -81,12 +81,23 @@ __global__ __launch_bounds__(NT) void scan_sums_single(T* sums, size_t nb, T* ou
         }
         carry += total;
     }
-    if (threadIdx.x == 0) *out_total = carry;
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void scan_tile_apply(T* data, size_t n, const T* sums) {
+// Exclusive scan of tile t.  Its base is either sums[t] (already scanned,
+// PRE = false) or, for scans of few tiles, the sum of the raw tile sums before
+// it reduced by this block (PRE = true, saving a launch).  The last tile
+// writes the total to data[n].
+template <typename T, bool PRE>
+__global__ __launch_bounds__(NT) void scan_tile_apply(T* data, size_t n, const T* sums, size_t nb) {
     __shared__ T lds[NT / 64 + 1];
+    T ptot;
+    if (PRE) {
+        T pre = 0;
+        for (size_t i = threadIdx.x; i < blockIdx.x; i += NT) pre += sums[i];
+        (void)block_exclusive_scan<T>(pre, lds, ptot);
+    } else {
+        ptot = sums[blockIdx.x];
+    }
     size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_ITEMS;
     T v[SC_ITEMS];
     T acc = 0;
@@ -96,12 +107,13 @@ __global__ __launch_bounds__(NT) void scan_tile_apply(T* data, size_t n, const T
         acc += v[k];
     }
     T total;
-    T ex = block_exclusive_scan<T>(acc, lds, total) + sums[blockIdx.x];
+    T ex = block_exclusive_scan<T>(acc, lds, total) + ptot;
 #pragma unroll
     for (int k = 0; k < SC_ITEMS; k++) {
         if (base + k < n) data[base + k] = ex;
         ex += v[k];
     }
+    if (blockIdx.x + 1 == nb && threadIdx.x == 0) data[n] = ptot + total;
 }
 
 template <typename T>
@@ -118,8 +130,14 @@ void exclusive_scan_impl(T* data, size_t n, T* sums, hipStream_t st) {
         return;
     }
     hipLaunchKernelGGL(scan_tile_sums<T>, dim3((unsigned)nb), dim3(NT), 0, st, data, n, sums);
-    hipLaunchKernelGGL(scan_sums_single<T>, dim3(1), dim3(NT), 0, st, sums, nb, data + n);
-    hipLaunchKernelGGL(scan_tile_apply<T>, dim3((unsigned)nb), dim3(NT), 0, st, data, n, sums);
+    if (nb <= 2048) {
+        hipLaunchKernelGGL((scan_tile_apply<T, true>), dim3((unsigned)nb), dim3(NT), 0, st, data, n,
+                           sums, nb);
+    } else {
+        hipLaunchKernelGGL(scan_sums_single<T>, dim3(1), dim3(NT), 0, st, sums, nb);
+        hipLaunchKernelGGL((scan_tile_apply<T, false>), dim3((unsigned)nb), dim3(NT), 0, st, data,
+                           n, sums, nb);
+    }
     CHK_HIP(hipGetLastError());
 }
 
