@@ -83,26 +83,56 @@ __device__ inline uint32_t bitrev(uint32_t i, uint32_t logm) {
     return logm ? __brev(i) >> (32 - logm) : 0u;
 }
 
-// In-place radix-2 DIT FFTs of nfft stacked length-M signals in LDS, input in
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// In-place DIT FFTs of nfft stacked length-M signals in LDS, input in
 // bit-reversed order, output in natural order.  tw[t] = exp(-2 pi i t / TW).
+// Stages are taken two at a time as radix-4 units held in registers (one LDS
+// round trip and one barrier per pair of radix-2 stages); an odd first stage
+// is a plain radix-2 pass.  Unit (k, group G) of stages s, s+1 (h = 2^(s-1)):
+// (a0..a3) = x[G + k + {0, h, 2h, 3h}], w1 = W_2h^k, w2 = W_4h^k,
+// stage s: b0,1 = a0 +- w1 a1, b2,3 = a2 +- w1 a3; stage s+1: c0,2 = b0 +- w2 b2,
+// c1,3 = b1 +- (w2 W_4h^h) b3 with W_4h^h = -i (+i for the inverse).
 template <bool INV>
 __device__ void lds_fft(double2* x, uint32_t M, uint32_t logM, uint32_t nfft, const double2* tw) {
-    const uint32_t nb = nfft * (M >> 1);
-    for (uint32_t s = 1; s <= logM; s++) {
-        const uint32_t half = 1u << (s - 1);
-        const uint32_t stride = TW >> s;  // TW / (2*half)
+    uint32_t s = 1;
+    if (logM & 1) {
+        const uint32_t nb = nfft * (M >> 1);
         for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
-            const uint32_t f = j >> (logM - 1);
-            const uint32_t jj = j & ((M >> 1) - 1);
-            const uint32_t k = jj & (half - 1);
-            const uint32_t i0 = f * M + ((jj >> (s - 1)) << s) + k;
-            const uint32_t i1 = i0 + half;
-            double2 w = tw[k * stride];
-            if (INV) w.y = -w.y;
-            const double2 a = x[i0], b = x[i1];
-            const double2 bw = {b.x * w.x - b.y * w.y, b.x * w.y + b.y * w.x};
-            x[i0] = {a.x + bw.x, a.y + bw.y};
-            x[i1] = {a.x - bw.x, a.y - bw.y};
+            const double2 a = x[2 * j], b = x[2 * j + 1];
+            x[2 * j] = {a.x + b.x, a.y + b.y};
+            x[2 * j + 1] = {a.x - b.x, a.y - b.y};
+        }
+        __syncthreads();
+        s = 2;
+    }
+    if (logM < 2) return;
+    const uint32_t nq = nfft * (M >> 2);
+    for (; s + 1 <= logM; s += 2) {
+        const uint32_t h = 1u << (s - 1);
+        const uint32_t st1 = TW >> s, st2 = TW >> (s + 1);
+        for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) {
+            const uint32_t f = j >> (logM - 2);
+            const uint32_t jj = j & ((M >> 2) - 1);
+            const uint32_t k = jj & (h - 1);
+            const uint32_t base = f * M + ((jj >> (s - 1)) << (s + 1)) + k;
+            double2 w1 = tw[k * st1], w2 = tw[k * st2];
+            if (INV) {
+                w1.y = -w1.y;
+                w2.y = -w2.y;
+            }
+            const double2 w3 = INV ? double2{-w2.y, w2.x} : double2{w2.y, -w2.x};  // w2 * (-+i)
+            const double2 a0 = x[base], a1 = x[base + h], a2 = x[base + 2 * h], a3 = x[base + 3 * h];
+            const double2 t1 = cmul(w1, a1), t3 = cmul(w1, a3);
+            const double2 b0 = {a0.x + t1.x, a0.y + t1.y}, b1 = {a0.x - t1.x, a0.y - t1.y};
+            const double2 b2 = {a2.x + t3.x, a2.y + t3.y}, b3 = {a2.x - t3.x, a2.y - t3.y};
+            const double2 u2 = cmul(w2, b2), u3 = cmul(w3, b3);
+            x[base] = {b0.x + u2.x, b0.y + u2.y};
+            x[base + 2 * h] = {b0.x - u2.x, b0.y - u2.y};
+            x[base + h] = {b1.x + u3.x, b1.y + u3.y};
+            x[base + 3 * h] = {b1.x - u3.x, b1.y - u3.y};
         }
         __syncthreads();
     }
