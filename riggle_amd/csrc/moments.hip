@@ -15,12 +15,15 @@
 //   mean(L) = S1/(100 n_w),  var(L) = (S2_H n_w - S1_H^2) / (1e4 n_w^2),
 //   S1_H = S1 - c L n_w, n_w = n - L + 1 (the variance is shift-invariant, so
 //   centring removes its cancellation).
-// The autocorrelation is one complex FFT of the zero-padded H of length
-// N = next_pow2(2n - 1), |X|^2, and the inverse FFT, written as LDS-staged
-// radix-2 passes: one block per signal when N fits in one block's LDS (8192
-// complex values where the device allows 128 KiB per block), otherwise the
-// four-step split N = N1 * N2 into column FFTs (length N2, twiddled), row
-// FFTs with |X|^2 and the inverse row FFTs fused, and inverse column FFTs.
+// The autocorrelation is a complex FFT of the zero-padded H of length
+// N = next_pow2(2n - 1), |X|^2, and the inverse FFT; two signals of the same
+// shard share each complex FFT (one as the real, one as the imaginary part).
+// The FFTs are LDS-staged radix-4 passes: one block per signal pair when N
+// fits in one block's LDS (8192 complex values where the device allows
+// 128 KiB per block), otherwise the four-step split N = N1 * N2 into column
+// FFTs (length N2, twiddled), row FFTs with the power spectrum and the
+// inverse row FFTs fused (a row and its mirror row per block), and inverse
+// column FFTs.
 // A is FP64; when every value is within 0.25 of an integer and A[0] equals
 // sum H^2 it is rounded to the exact integers.  Prefix sums are double-double
 // (two-sum compensated), so S2 keeps ~32 significant digits.  Tables are
@@ -34,8 +37,10 @@ namespace chk {
 namespace {
 
 constexpr int NT = 256;
-constexpr uint32_t FFT_MAXM = 8192;  // largest LDS-resident FFT (complex FP64)
-constexpr uint32_t TW = FFT_MAXM;    // twiddle table period
+constexpr int NTF = 1024;            // threads of the FFT kernels (16 waves beside one LDS tile)
+constexpr uint32_t LOG_MAXM = 13;
+constexpr uint32_t FFT_MAXM = 1u << LOG_MAXM;  // largest LDS-resident FFT (complex FP64)
+constexpr uint32_t TW = FFT_MAXM;              // twiddle table period
 
 // ------------------------------------------------------------ double-double
 struct dd {
@@ -138,128 +143,198 @@ __device__ void lds_fft(double2* x, uint32_t M, uint32_t logM, uint32_t nfft, co
     }
 }
 
-// exp(-+2 pi i e / N) for e < N (four-step twiddles)
-__device__ inline double2 twiddle(uint64_t e, uint64_t N, bool inv) {
-    double s, c;
-    sincospi(-2.0 * (double)e / (double)N, &s, &c);
-    return {c, inv ? -s : s};
-}
-
 struct Rows {
-    const int32_t* G;      // [B][ldg] coverage per bin
+    const int32_t* G;      // [nb][ldg] coverage per bin
     uint32_t ldg;
-    const int64_t* cen;    // [B] centring constant c
-    double2* X;            // [B][N] FFT workspace
-    double* A;             // [B][n] autocorrelation output
-    uint32_t n, N, logN;
-    const double2* tw;
+    const int64_t* cen;    // [nb] centring constant c
+    double2* X;            // [(nb + 1) / 2][N] FFT workspace, one signal pair per row
+    double* A;             // [nb][n] autocorrelation output
+    uint32_t nb, n, N, logN;
+    const double2* tw;     // exp(-2 pi i t / TW), t < TW
+    const double2* twN;    // exp(-2 pi i t / N), t < FFT_MAXM (four-step only)
 };
 
-// Whole autocorrelation of one signal in LDS (N <= FFT_MAXM).
-__global__ __launch_bounds__(NT) void autocorr_small(Rows r) {
-    extern __shared__ __align__(16) double2 xs[];
-    const uint32_t b = blockIdx.x;
-    const int32_t* G = r.G + (size_t)b * r.ldg;
-    const double c = (double)r.cen[b];
-    for (uint32_t i = threadIdx.x; i < r.N; i += NT)
-        xs[bitrev(i, r.logN)] = {i < r.n ? (double)G[i] - c : 0.0, 0.0};
-    __syncthreads();
-    lds_fft<false>(xs, r.N, r.logN, 1, r.tw);
-    for (uint32_t i = threadIdx.x; i < r.N; i += NT) {
-        const double2 v = xs[i];
-        xs[i] = {v.x * v.x + v.y * v.y, 0.0};
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < r.N; i += NT) {  // back to bit-reversed order
-        const uint32_t j = bitrev(i, r.logN);
-        if (i < j) {
-            const double2 t = xs[i];
-            xs[i] = xs[j];
-            xs[j] = t;
-        }
-    }
-    __syncthreads();
-    lds_fft<true>(xs, r.N, r.logN, 1, r.tw);
-    double* A = r.A + (size_t)b * r.n;
-    for (uint32_t i = threadIdx.x; i < r.n; i += NT) A[i] = xs[i].x / (double)r.N;
+// exp(-+2 pi i e / N) for e < N <= 2^(2 LOG_MAXM): W_N^(e_hi 2^LOG_MAXM) from the
+// TW table (it is W_(N / 2^LOG_MAXM)^e_hi) times W_N^e_lo from the per-N table.
+__device__ inline double2 twiddle(const Rows& r, uint64_t e, bool inv) {
+    const double2 hi = r.tw[(e >> LOG_MAXM) << (2 * LOG_MAXM - r.logN)];
+    const double2 lo = r.twN[e & (FFT_MAXM - 1)];
+    double2 w = cmul(hi, lo);
+    if (inv) w.y = -w.y;
+    return w;
 }
 
-// Four-step pass 1: CB adjacent columns n1 of one signal, length-N2 FFTs over
-// n2 (element n1 + N1*n2), twiddle W_N^(n1 k2), stored at n1 + N1*k2.
-__global__ __launch_bounds__(NT) void fft_cols_fwd(Rows r, uint32_t N1, uint32_t N2,
-                                                   uint32_t logN2, uint32_t CB) {
+// Two real signals share one complex FFT: z = h1 + i h2 with h = G - c of
+// rows 2p and 2p + 1 (zero past n and for a missing odd row).  From a = Z[k],
+// b = Z[N - k]: H1 = (a + conj b) / 2, H2 = (a - conj b) / 2i, and
+// Y[k] = |H1|^2 + i |H2|^2 = Y[N - k]; the inverse FFT of Y is A1 + i A2,
+// both autocorrelations being real.
+__device__ inline double2 pair_power(double2 a, double2 b) {
+    const double sr = a.x + b.x, si = a.y - b.y;
+    const double dr = a.x - b.x, di = a.y + b.y;
+    return {0.25 * (sr * sr + si * si), 0.25 * (dr * dr + di * di)};
+}
+
+struct PairIn {
+    const int32_t* g0;
+    const int32_t* g1;  // nullptr for a missing odd row
+    double c0, c1;
+    uint32_t n;
+    __device__ PairIn(const Rows& r, uint32_t p) {
+        g0 = r.G + (size_t)(2 * p) * r.ldg;
+        c0 = (double)r.cen[2 * p];
+        const bool two = 2 * p + 1 < r.nb;
+        g1 = two ? g0 + r.ldg : nullptr;
+        c1 = two ? (double)r.cen[2 * p + 1] : 0.0;
+        n = r.n;
+    }
+    __device__ double2 operator()(uint64_t i) const {
+        if (i >= n) return {0.0, 0.0};
+        return {(double)g0[i] - c0, g1 ? (double)g1[i] - c1 : 0.0};
+    }
+};
+
+struct PairOut {
+    double* a0;
+    double* a1;  // nullptr for a missing odd row
+    double scale;
+    __device__ PairOut(const Rows& r, uint32_t p) {
+        a0 = r.A + (size_t)(2 * p) * r.n;
+        a1 = 2 * p + 1 < r.nb ? a0 + r.n : nullptr;
+        scale = 1.0 / (double)r.N;
+    }
+    __device__ void operator()(uint64_t i, double2 v) const {
+        a0[i] = v.x * scale;
+        if (a1) a1[i] = v.y * scale;
+    }
+};
+
+// Natural order -> bit-reversed order of nfft stacked length-M rows in LDS.
+__device__ inline void lds_bitrev(double2* xs, uint32_t M, uint32_t logM, uint32_t nfft) {
+    for (uint32_t t = threadIdx.x; t < nfft * M; t += blockDim.x) {
+        const uint32_t i = t & (M - 1), j = bitrev(i, logM);
+        if (i < j) {
+            const uint32_t o = t - i;
+            const double2 v = xs[o + i];
+            xs[o + i] = xs[o + j];
+            xs[o + j] = v;
+        }
+    }
+}
+
+// Whole autocorrelation of one signal pair in LDS (N <= FFT_MAXM).
+__global__ __launch_bounds__(NTF) void autocorr_small(Rows r) {
+    extern __shared__ __align__(16) double2 xs[];
+    const uint32_t p = blockIdx.x;
+    const PairIn in(r, p);
+    for (uint32_t i = threadIdx.x; i < r.N; i += NTF) xs[bitrev(i, r.logN)] = in(i);
+    __syncthreads();
+    lds_fft<false>(xs, r.N, r.logN, 1, r.tw);
+    for (uint32_t k = threadIdx.x; k <= r.N / 2; k += NTF) {
+        const uint32_t kk = (r.N - k) & (r.N - 1);
+        const double2 y = pair_power(xs[k], xs[kk]);
+        xs[k] = y;
+        xs[kk] = y;
+    }
+    __syncthreads();
+    lds_bitrev(xs, r.N, r.logN, 1);
+    __syncthreads();
+    lds_fft<true>(xs, r.N, r.logN, 1, r.tw);
+    const PairOut out(r, p);
+    for (uint32_t i = threadIdx.x; i < r.n; i += NTF) out(i, xs[i]);
+}
+
+// Four-step pass 1: CB adjacent columns n1 of one signal pair, length-N2 FFTs
+// over n2 (element n1 + N1*n2), twiddle W_N^(n1 k2), stored at n1 + N1*k2.
+__global__ __launch_bounds__(NTF) void fft_cols_fwd(Rows r, uint32_t N1, uint32_t N2,
+                                                    uint32_t logN2, uint32_t CB) {
     extern __shared__ __align__(16) double2 xs[];
     const uint32_t ncb = N1 / CB;
-    const uint32_t b = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
-    const int32_t* G = r.G + (size_t)b * r.ldg;
-    const double c = (double)r.cen[b];
-    for (uint32_t t = threadIdx.x; t < CB * N2; t += NT) {
+    const uint32_t p = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
+    const PairIn in(r, p);
+    for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
         const uint32_t cc = t % CB, n2 = t / CB;
-        const uint64_t idx = (uint64_t)(c0 + cc) + (uint64_t)N1 * n2;
-        xs[cc * N2 + bitrev(n2, logN2)] = {idx < r.n ? (double)G[idx] - c : 0.0, 0.0};
+        xs[cc * N2 + bitrev(n2, logN2)] = in((uint64_t)(c0 + cc) + (uint64_t)N1 * n2);
     }
     __syncthreads();
     lds_fft<false>(xs, N2, logN2, CB, r.tw);
-    double2* X = r.X + (size_t)b * r.N;
-    for (uint32_t t = threadIdx.x; t < CB * N2; t += NT) {
+    double2* X = r.X + (size_t)p * r.N;
+    for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
         const uint32_t cc = t % CB, k2 = t / CB;
         const uint32_t n1 = c0 + cc;
-        const double2 v = xs[cc * N2 + k2];
-        const double2 w = twiddle(((uint64_t)n1 * k2) % r.N, r.N, false);
-        X[n1 + (size_t)N1 * k2] = {v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x};
+        X[n1 + (size_t)N1 * k2] = cmul(xs[cc * N2 + k2], twiddle(r, (uint64_t)n1 * k2, false));
     }
 }
 
-// Four-step pass 2: one row k2 (contiguous N1): FFT, |X|^2, inverse FFT,
-// twiddle W_N^(-n1 k2), in place.
-__global__ __launch_bounds__(NT) void fft_rows(Rows r, uint32_t N1, uint32_t logN1, uint32_t N2) {
+// Four-step pass 2: the rows k2 and N2 - k2 of one signal pair (row k2 holds
+// Z[k2 + N2 k1], so Z[N - k] of its entry k1 is entry N1 - 1 - k1 of row
+// N2 - k2; rows 0 and N2/2 pair with themselves): forward FFTs, Y, inverse
+// FFTs, twiddle W_N^(-n1 k2), in place.
+__global__ __launch_bounds__(NTF) void fft_rows(Rows r, uint32_t N1, uint32_t logN1, uint32_t N2) {
     extern __shared__ __align__(16) double2 xs[];
-    const uint32_t b = blockIdx.x / N2, k2 = blockIdx.x % N2;
-    double2* X = r.X + (size_t)b * r.N + (size_t)k2 * N1;
-    for (uint32_t i = threadIdx.x; i < N1; i += NT) xs[bitrev(i, logN1)] = X[i];
-    __syncthreads();
-    lds_fft<false>(xs, N1, logN1, 1, r.tw);
-    for (uint32_t i = threadIdx.x; i < N1; i += NT) {
-        const double2 v = xs[i];
-        xs[i] = {v.x * v.x + v.y * v.y, 0.0};
+    const uint32_t per = N2 / 2 + 1;
+    const uint32_t p = blockIdx.x / per, q = blockIdx.x % per;
+    const uint32_t nr = (q == 0 || q == N2 / 2) ? 1 : 2;
+    double2* X = r.X + (size_t)p * r.N;
+    for (uint32_t t = threadIdx.x; t < nr * N1; t += NTF) {
+        const uint32_t j = t >> logN1, i = t & (N1 - 1);
+        const uint32_t row = j ? N2 - q : q;
+        xs[j * N1 + bitrev(i, logN1)] = X[(size_t)row * N1 + i];
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < N1; i += NT) {
-        const uint32_t j = bitrev(i, logN1);
-        if (i < j) {
-            const double2 t = xs[i];
-            xs[i] = xs[j];
-            xs[j] = t;
+    lds_fft<false>(xs, N1, logN1, nr, r.tw);
+    if (nr == 2) {
+        for (uint32_t k = threadIdx.x; k < N1; k += NTF) {
+            const uint32_t kk = N1 + N1 - 1 - k;
+            const double2 y = pair_power(xs[k], xs[kk]);
+            xs[k] = y;
+            xs[kk] = y;
+        }
+    } else if (q == 0) {
+        for (uint32_t k = threadIdx.x; k <= N1 / 2; k += NTF) {
+            const uint32_t kk = (N1 - k) & (N1 - 1);
+            const double2 y = pair_power(xs[k], xs[kk]);
+            xs[k] = y;
+            xs[kk] = y;
+        }
+    } else {
+        for (uint32_t k = threadIdx.x; k < N1 / 2; k += NTF) {
+            const uint32_t kk = N1 - 1 - k;
+            const double2 y = pair_power(xs[k], xs[kk]);
+            xs[k] = y;
+            xs[kk] = y;
         }
     }
     __syncthreads();
-    lds_fft<true>(xs, N1, logN1, 1, r.tw);
-    for (uint32_t n1 = threadIdx.x; n1 < N1; n1 += NT) {
-        const double2 v = xs[n1];
-        const double2 w = twiddle(((uint64_t)n1 * k2) % r.N, r.N, true);
-        X[n1] = {v.x * w.x - v.y * w.y, v.x * w.y + v.y * w.x};
+    lds_bitrev(xs, N1, logN1, nr);
+    __syncthreads();
+    lds_fft<true>(xs, N1, logN1, nr, r.tw);
+    for (uint32_t t = threadIdx.x; t < nr * N1; t += NTF) {
+        const uint32_t j = t >> logN1, n1 = t & (N1 - 1);
+        const uint32_t row = j ? N2 - q : q;
+        X[(size_t)row * N1 + n1] = cmul(xs[t], twiddle(r, (uint64_t)n1 * row, true));
     }
 }
 
 // Four-step pass 3: CB columns, inverse length-N2 FFTs over k2, natural n2;
-// A[n1 + N1*n2] = Re / N for indices below n.
-__global__ __launch_bounds__(NT) void fft_cols_inv(Rows r, uint32_t N1, uint32_t N2,
-                                                   uint32_t logN2, uint32_t CB) {
+// A[n1 + N1*n2] = Re / N (first row), Im / N (second row) below n.
+__global__ __launch_bounds__(NTF) void fft_cols_inv(Rows r, uint32_t N1, uint32_t N2,
+                                                    uint32_t logN2, uint32_t CB) {
     extern __shared__ __align__(16) double2 xs[];
     const uint32_t ncb = N1 / CB;
-    const uint32_t b = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
-    const double2* X = r.X + (size_t)b * r.N;
-    for (uint32_t t = threadIdx.x; t < CB * N2; t += NT) {
+    const uint32_t p = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
+    const double2* X = r.X + (size_t)p * r.N;
+    for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
         const uint32_t cc = t % CB, k2 = t / CB;
         xs[cc * N2 + bitrev(k2, logN2)] = X[(c0 + cc) + (size_t)N1 * k2];
     }
     __syncthreads();
     lds_fft<true>(xs, N2, logN2, CB, r.tw);
-    double* A = r.A + (size_t)b * r.n;
-    for (uint32_t t = threadIdx.x; t < CB * N2; t += NT) {
+    const PairOut out(r, p);
+    for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
         const uint32_t cc = t % CB, n2 = t / CB;
         const uint64_t idx = (uint64_t)(c0 + cc) + (uint64_t)N1 * n2;
-        if (idx < r.n) A[idx] = xs[cc * N2 + n2].x / (double)r.N;
+        if (idx < r.n) out(idx, xs[cc * N2 + n2]);
     }
 }
 
@@ -565,15 +640,16 @@ void moments_build(Index& ix) {
         }
     M.mean.reserve(total);
     M.var.reserve(total);
-    // twiddles exp(-2 pi i t / TW), t < TW/2, from the host C library in FP64
-    std::vector<double2> twh(TW / 2);
-    for (uint32_t t = 0; t < TW / 2; t++) {
+    // twiddles exp(-2 pi i t / TW), t < TW, from the host C library in FP64
+    std::vector<double2> twh(TW), twnh(FFT_MAXM);
+    for (uint32_t t = 0; t < TW; t++) {
         const double ang = -2.0 * M_PI * (double)t / (double)TW;
         twh[t] = make_double2(std::cos(ang), std::sin(ang));
     }
-    DevBuf<double2> tw;
-    tw.reserve(TW / 2);
-    CHK_HIP(hipMemcpyAsync(tw.p, twh.data(), (TW / 2) * sizeof(double2), hipMemcpyHostToDevice, st));
+    DevBuf<double2> tw, twN;
+    tw.reserve(TW);
+    twN.reserve(FFT_MAXM);
+    CHK_HIP(hipMemcpyAsync(tw.p, twh.data(), TW * sizeof(double2), hipMemcpyHostToDevice, st));
     DevBuf<int32_t> slot_of_fid, pmapL, pmapM, D;
     DevBuf<int64_t> cen, sumg, out_off, sPP_L, sPP_M;
     DevBuf<dd> sC1, sC2, sE1, sQ;
@@ -608,6 +684,14 @@ void moments_build(Index& ix) {
         CHK_REQUIRE(N <= (uint64_t)maxm * maxm / 8, CHK_ERR_INVALID,
                     "shard too long for the moment FFT");
         const uint32_t logN = ilog2(N);
+        if (N > maxm) {  // four-step twiddles exp(-2 pi i t / N), t < FFT_MAXM
+            CHK_HIP(hipStreamSynchronize(st));  // the previous shard's table is no longer read
+            for (uint32_t t = 0; t < FFT_MAXM; t++) {
+                const double ang = -2.0 * M_PI * (double)t / (double)N;
+                twnh[t] = make_double2(std::cos(ang), std::sin(ang));
+            }
+            CHK_HIP(hipMemcpyAsync(twN.p, twnh.data(), FFT_MAXM * sizeof(double2), hipMemcpyHostToDevice, st));
+        }
         // sample maps: position L -> index of L; position n-L+1 -> index of L
         std::vector<int32_t> pl(n + 2, -1), pm(n + 2, -1);
         for (uint32_t j = 0; j < nl; j++) {
@@ -625,12 +709,12 @@ void moments_build(Index& ix) {
         // give every CU rows to work on
         size_t free_b = 0, total_b = 0;
         CHK_HIP(hipMemGetInfo(&free_b, &total_b));
-        const uint64_t row_bytes = N * sizeof(double2) + (uint64_t)n * 12 + 64 * (uint64_t)nl;
+        const uint64_t row_bytes = N * sizeof(double2) / 2 + (uint64_t)n * 12 + 64 * (uint64_t)nl;
         const uint32_t B = (uint32_t)std::max<uint64_t>(
             1, std::min<uint64_t>({files.size(), (uint64_t)(free_b / 3) / row_bytes, 4096}));
         const uint32_t ldg = n + 1;
         D.reserve((size_t)B * ldg);
-        X.reserve((size_t)B * N);
+        X.reserve((size_t)(B + 1) / 2 * N);
         A.reserve((size_t)B * n);
         cen.reserve(B);
         sumg.reserve(B);
@@ -662,19 +746,21 @@ void moments_build(Index& ix) {
                                    ldg, D.p);
             hipLaunchKernelGGL(coverage_rows, dim3(nb), dim3(NT), 0, st, D.p, ldg, n, cen.p, sumg.p);
             CHK_HIP(hipGetLastError());
-            Rows r{D.p, ldg, cen.p, X.p, A.p, n, (uint32_t)N, logN, tw.p};
+            Rows r{D.p, ldg, cen.p, X.p, A.p, nb, n, (uint32_t)N, logN, tw.p, twN.p};
+            const uint32_t np = (nb + 1) / 2;  // signal pairs
             hipEvent_t t0 = ix.prof.begin(st);
             if (N <= maxm) {
-                hipLaunchKernelGGL(autocorr_small, dim3(nb), dim3(NT), N * sizeof(double2), st, r);
+                hipLaunchKernelGGL(autocorr_small, dim3(np), dim3(NTF), N * sizeof(double2), st, r);
             } else {
-                const uint32_t N1 = maxm, N2 = (uint32_t)(N / N1);
+                // rows of N1 = maxm / 2 (two per block), columns of N2 = N / N1 (CB per block)
+                const uint32_t N1 = maxm / 2, N2 = (uint32_t)(N / N1);
                 const uint32_t CB = std::min<uint32_t>(N1, maxm / N2);
                 const size_t lds_col = (size_t)CB * N2 * sizeof(double2);
-                hipLaunchKernelGGL(fft_cols_fwd, dim3(nb * (N1 / CB)), dim3(NT), lds_col, st, r, N1, N2,
+                hipLaunchKernelGGL(fft_cols_fwd, dim3(np * (N1 / CB)), dim3(NTF), lds_col, st, r, N1, N2,
                                    ilog2(N2), CB);
-                hipLaunchKernelGGL(fft_rows, dim3(nb * N2), dim3(NT), N1 * sizeof(double2), st, r, N1,
-                                   ilog2(N1), N2);
-                hipLaunchKernelGGL(fft_cols_inv, dim3(nb * (N1 / CB)), dim3(NT), lds_col, st, r, N1, N2,
+                hipLaunchKernelGGL(fft_rows, dim3(np * (N2 / 2 + 1)), dim3(NTF), 2 * N1 * sizeof(double2),
+                                   st, r, N1, ilog2(N1), N2);
+                hipLaunchKernelGGL(fft_cols_inv, dim3(np * (N1 / CB)), dim3(NTF), lds_col, st, r, N1, N2,
                                    ilog2(N2), CB);
             }
             CHK_HIP(hipGetLastError());
