@@ -155,8 +155,12 @@ int chk_sync(chk_index* idx);
  * Kernel timing with HIP events recorded on the index's stream around each
  * named kernel ("sweep", "partition", "qprep", ...).  chk_profile_read
  * synchronises and returns the number of launches and their summed duration.
+ * chk_profile_only restricts recording to a comma-separated list of region
+ * names (NULL or "" records every region); each recorded region costs the
+ * stream a few microseconds.
  */
 int chk_profile_enable(chk_index* idx, int on);
+int chk_profile_only(chk_index* idx, const char* regions);
 int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, double* total_ms);
 
 /*
@@ -196,9 +200,10 @@ int chk_autotune(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_b
  * zeroing, event sort, prefix sums, merge-path partition, the persistent
  * work-queue dense sweep and the sparse pair sweep) and, when d_p is given,
  * the p-value kernel, captured on the index's stream.  Replays need no host
- * work and no allocation.  The graph reads the query's device buffers, so
- * it sees new contents uploaded into them; it must be recreated after the
- * index's tuning changes (chk_graph_launch then reports an error).
+ * work and no allocation.  The graph replays the batch it was created for
+ * (its launch plan -- tiles, chunks, event key widths -- is that batch's); a
+ * new batch takes a new query and graph.  It must also be recreated after
+ * the index's tuning changes (chk_graph_launch then reports an error).
  */
 typedef struct chk_graph chk_graph;
 int chk_graph_create(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_bp,

@@ -60,6 +60,7 @@ def _declare(L):
     L.chk_dev_to_host.argtypes = [vp, vp, vp, u64]
     L.chk_sync.argtypes = [vp]
     L.chk_profile_enable.argtypes = [vp, i32]
+    L.chk_profile_only.argtypes = [vp, C.c_char_p]
     L.chk_profile_read.argtypes = [vp, C.c_char_p, C.POINTER(u64), C.POINTER(C.c_double)]
     L.chk_calibrate_copy.argtypes = [vp, u64]
     L.chk_tune.argtypes = [vp, C.c_char_p, C.c_int64]
@@ -145,7 +146,9 @@ class Index:
     def sync(self):
         _check(lib().chk_sync(self._h))
 
-    def profile(self, on=True):
+    def profile(self, on=True, only=()):
+        """Record kernel timings (only the named regions when `only` is given)."""
+        _check(lib().chk_profile_only(self._h, ",".join(only).encode()))
         _check(lib().chk_profile_enable(self._h, 1 if on else 0))
 
     def profile_read(self, name):

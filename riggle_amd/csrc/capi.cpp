@@ -322,6 +322,26 @@ int chk_profile_enable(chk_index* idx, int on) {
     });
 }
 
+int chk_profile_only(chk_index* idx, const char* regions) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");
+        std::set<std::string> only;
+        if (regions) {
+            std::string cur;
+            for (const char* c = regions;; c++) {
+                if (*c == ',' || *c == 0) {
+                    if (!cur.empty()) only.insert(cur);
+                    cur.clear();
+                    if (!*c) break;
+                } else {
+                    cur.push_back(*c);
+                }
+            }
+        }
+        idx->ix.prof.only = only;
+    });
+}
+
 int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, double* total_ms) {
     return guarded([&] {
         CHK_REQUIRE(idx && name, CHK_ERR_INVALID, "NULL argument");

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <map>
+#include <set>
 #include <string>
 #include <utility>
 
@@ -14,6 +15,7 @@ namespace chk {
 // HIP-event timing of named kernels on one stream.
 struct Profiler {
     bool on = false;
+    std::set<std::string> only;  // regions to record; empty = all
     std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> ev;
     std::vector<hipEvent_t> pool;
     hipEvent_t get() {
@@ -26,9 +28,9 @@ struct Profiler {
         CHK_HIP(hipEventCreate(&e));
         return e;
     }
-    // Returns the start event (or nullptr when profiling is off).
-    hipEvent_t begin(hipStream_t st) {
-        if (!on) return nullptr;
+    // Returns the start event (or nullptr when the region is not recorded).
+    hipEvent_t begin(const char* name, hipStream_t st) {
+        if (!on || (!only.empty() && !only.count(name))) return nullptr;
         hipEvent_t a = get();
         CHK_HIP(hipEventRecord(a, st));
         return a;
@@ -164,6 +166,7 @@ struct Query {
     uint32_t nfiles = 0;
     hipStream_t stream = nullptr;     // uploads and plan copies (the index's stream by default)
     uint64_t n_in = 0;                // intervals given
+    uint32_t max_end = 0;             // largest end coordinate kept (sizes the event keys)
     uint64_t n = 0;                   // intervals kept (on shards of the DB)
     std::vector<double> file_len;     // total length per query file
     std::vector<double> file_visits;  // estimated sparse-path visits per query file
@@ -197,7 +200,6 @@ struct Query {
     DevBuf<uint32_t> key32;           // their low 32 bits (2*coord + is_start)
     DevBuf<uint32_t> rs;              // exclusive prefix: # start events
     DevBuf<int64_t> w;                // exclusive prefix: sum(start) - sum(end)
-    DevBuf<int64_t> w_sums;
     SortScratch scratch;              // per query, so batches on other streams do not share it
     // query length groups per (query file, shard), built when the index has a
     // genome table (src/fourier/moments.rs:138-177)

@@ -104,33 +104,6 @@ __global__ void build_jump(const LayerSeg* segs, uint32_t nseg, const int32_t* l
     }
 }
 
-// Two events per interval of a dense query file.
-__global__ void make_event_keys(const uint32_t* segid, const uint32_t* start, const uint32_t* end,
-                                size_t n, uint64_t* keys) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t seg = (uint64_t)segid[i] << 32;
-    keys[2 * i] = seg | (2ull * start[i] + 1ull);
-    keys[2 * i + 1] = seg | (2ull * end[i]);
-}
-
-// Per-event values to be prefix-summed and the 32-bit keys; entry m gets the
-// sentinel key above every DB key.
-__global__ void event_values(const uint64_t* keys, size_t m, uint32_t* rs, int64_t* w,
-                             uint32_t* key32) {
-    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > m) return;
-    if (k == m) {
-        key32[m] = 0xFFFFFFFFu;
-        return;
-    }
-    const uint32_t lo = (uint32_t)keys[k];
-    key32[k] = lo;
-    rs[k] = lo & 1u;
-    const int64_t c = (int64_t)(lo >> 1);
-    w[k] = (lo & 1u) ? c : -c;
-}
-
 __global__ void fill_u64(uint64_t* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -284,6 +257,7 @@ void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const 
     q.file_n.assign(R, 0);
     q.file_len.assign(R, 0.0);
     q.fs_n.assign((size_t)R * S, 0);
+    q.max_end = 0;
     for (uint64_t i = 0; i < n_in; i++) {
         CHK_REQUIRE(start[i] < end[i], CHK_ERR_INVALID, "query interval start must be less than end");
         CHK_REQUIRE(end[i] <= 0x7FFFFFFFu, CHK_ERR_INVALID,
@@ -292,6 +266,7 @@ void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const 
         CHK_REQUIRE(r < R, CHK_ERR_INVALID, "query file id out of range");
         if (shard[i] >= S || !ix.shard_n[shard[i]]) continue;
         q.file_n[r]++;
+        q.max_end = std::max(q.max_end, end[i]);
         q.file_len[r] += (double)(end[i] - start[i]);
         q.fs_n[(size_t)r * S + shard[i]]++;
     }
@@ -515,18 +490,9 @@ void query_place(Query& q) {
 // Sort the dense files' events and build the prefix sums (enqueued on st).
 void query_events(Query& q, hipStream_t st) {
     Index& ix = *q.idx;
-    const size_t m = 2 * q.n_dense;
-    if (m) {
-        hipLaunchKernelGGL(make_event_keys, dim3(nblk(q.n_dense)), dim3(NT), 0, st, q.segid.p,
-                           q.start.p, q.end.p, (size_t)q.n_dense, q.keys.p);
-        const uint32_t nseg = (uint32_t)q.dense_rows.size() * ix.nshards;
-        radix_sort_u64(q.keys.p, nullptr, m, 32 + bits_for(nseg - 1), q.scratch, st);
-    }
-    hipLaunchKernelGGL(event_values, dim3(nblk(m + 1)), dim3(NT), 0, st, q.keys.p, m, q.rs.p, q.w.p,
-                       q.key32.p);
-    CHK_HIP(hipGetLastError());
-    exclusive_scan_u32(q.rs.p, m, q.scratch, st);
-    exclusive_scan_i64(q.w.p, m, q.w_sums, st);
+    const uint32_t nseg = std::max<uint32_t>(1, (uint32_t)q.dense_rows.size() * ix.nshards);
+    sort_events(q.segid.p, q.start.p, q.end.p, q.n_dense, bits_for(nseg - 1),
+                bits_for(2ull * q.max_end), q.key32.p, q.rs.p, q.w.p, q.keys.p, q.scratch, st);
 }
 
 }  // namespace chk

@@ -748,7 +748,7 @@ void moments_build(Index& ix) {
             CHK_HIP(hipGetLastError());
             Rows r{D.p, ldg, cen.p, X.p, A.p, nb, n, (uint32_t)N, logN, tw.p, twN.p};
             const uint32_t np = (nb + 1) / 2;  // signal pairs
-            hipEvent_t t0 = ix.prof.begin(st);
+            hipEvent_t t0 = ix.prof.begin("moments_fft", st);
             if (N <= maxm) {
                 hipLaunchKernelGGL(autocorr_small, dim3(np), dim3(NTF), N * sizeof(double2), st, r);
             } else {
@@ -767,7 +767,7 @@ void moments_build(Index& ix) {
             ix.prof.end("moments_fft", t0, st);
             EvalArgs ea{D.p, ldg, n, nl, A.p, cen.p, sumg.p, pmapL.p, pmapM.p, d_ls.p, out_off.p,
                         M.mean.p, M.var.p, sPP_L.p, sPP_M.p, sC1.p, sC2.p, sE1.p, sQ.p, d_exact.p};
-            t0 = ix.prof.begin(st);
+            t0 = ix.prof.begin("moments_eval", st);
             hipLaunchKernelGGL(eval_rows, dim3(nb), dim3(NT), 0, st, ea);
             CHK_HIP(hipGetLastError());
             ix.prof.end("moments_eval", t0, st);

@@ -33,6 +33,7 @@ struct Request {
     std::vector<uint32_t> shard, start, end;  // on shards of the DB only
     std::vector<uint64_t> fs_n;               // intervals per shard
     double total_len = 0.0;
+    uint32_t max_end = 0;
     uint64_t n_given = 0;
     uint64_t footprint = 0;
     Clock::time_point t_submit, t_done;
@@ -131,8 +132,10 @@ struct Server {
         q.file_len.assign(B, 0.0);
         q.fs_n.assign((size_t)B * S, 0);
         q.file_off.assign(B + 1, 0);
+        q.max_end = 0;
         for (uint32_t b = 0; b < B; b++) {
             const Request* r = s.reqs[b];
+            q.max_end = std::max(q.max_end, r->max_end);
             q.file_n[b] = r->start.size();
             q.file_len[b] = r->total_len;
             std::copy(r->fs_n.begin(), r->fs_n.end(), q.fs_n.begin() + (size_t)b * S);
@@ -308,6 +311,7 @@ struct Server {
             r->start[k] = start[i];
             r->end[k] = end[i];
             r->fs_n[shard[i]]++;
+            r->max_end = std::max(r->max_end, end[i]);
             r->total_len += (double)(end[i] - start[i]);
             k++;
         }

@@ -43,16 +43,50 @@ __device__ T block_exclusive_scan(T v, T* lds, T& total) {
     return res;
 }
 
+// A thread's run of SC_ITEMS consecutive elements, read and written as 16-byte
+// vectors when the whole run is in range (runs start 16-byte aligned), else
+// element by element with zeros past n.
+template <typename T>
+__device__ inline void load_run(const T* p, size_t base, size_t n, T (&v)[SC_ITEMS]) {
+    static_assert((sizeof(T) * SC_ITEMS) % 16 == 0, "run must be whole 16-byte vectors");
+    if (base + SC_ITEMS <= n) {
+        const uint4* q = reinterpret_cast<const uint4*>(p + base);
+        uint4 t[sizeof(T) * SC_ITEMS / 16];
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) * SC_ITEMS / 16); i++) t[i] = q[i];
+        __builtin_memcpy(v, t, sizeof(v));
+    } else {
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; k++) v[k] = base + k < n ? p[base + k] : T(0);
+    }
+}
+
+template <typename T>
+__device__ inline void store_run(T* p, size_t base, size_t n, const T (&v)[SC_ITEMS]) {
+    if (base + SC_ITEMS <= n) {
+        uint4 t[sizeof(T) * SC_ITEMS / 16];
+        __builtin_memcpy(t, v, sizeof(v));
+        uint4* q = reinterpret_cast<uint4*>(p + base);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) * SC_ITEMS / 16); i++) q[i] = t[i];
+    } else {
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; k++)
+            if (base + k < n) p[base + k] = v[k];
+    }
+}
+
 // ------------------------------------------------------------------ scan
 
 template <typename T>
 __global__ __launch_bounds__(NT) void scan_tile_sums(const T* data, size_t n, T* sums) {
     __shared__ T lds[NT / 64 + 1];
     size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_ITEMS;
+    T v[SC_ITEMS];
+    load_run<T>(data, base, n, v);
     T acc = 0;
 #pragma unroll
-    for (int k = 0; k < SC_ITEMS; k++)
-        if (base + k < n) acc += data[base + k];
+    for (int k = 0; k < SC_ITEMS; k++) acc += v[k];
     T total;
     (void)block_exclusive_scan<T>(acc, lds, total);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
@@ -100,19 +134,19 @@ __global__ __launch_bounds__(NT) void scan_tile_apply(T* data, size_t n, const T
     }
     size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_ITEMS;
     T v[SC_ITEMS];
+    load_run<T>(data, base, n, v);
     T acc = 0;
 #pragma unroll
-    for (int k = 0; k < SC_ITEMS; k++) {
-        v[k] = base + k < n ? data[base + k] : T(0);
-        acc += v[k];
-    }
+    for (int k = 0; k < SC_ITEMS; k++) acc += v[k];
     T total;
     T ex = block_exclusive_scan<T>(acc, lds, total) + ptot;
 #pragma unroll
     for (int k = 0; k < SC_ITEMS; k++) {
-        if (base + k < n) data[base + k] = ex;
-        ex += v[k];
+        const T t = v[k];
+        v[k] = ex;
+        ex += t;
     }
+    store_run<T>(data, base, n, v);
     if (blockIdx.x + 1 == nb && threadIdx.x == 0) data[n] = ptot + total;
 }
 
@@ -206,7 +240,310 @@ __global__ __launch_bounds__(NT) void rs_scatter(const uint64_t* kin, const uint
     }
 }
 
+// ----------------------------------------------------------- event sort
+//
+// Sort of the 2n events of n query intervals by the key (segment, 2x + b):
+// a start event has b = 1 at x = start, an end event b = 0 at x = end, so on
+// equal coordinates ends come first.  LSD passes of at most EV_MAXD bits over
+// the key bits the batch actually uses.  The first pass builds the keys from
+// the intervals as it reads them; the last pass writes the per-event values
+// (key32 = 2x + b, rs = b, w = +-x) instead of the keys.  Scatter ranking is
+// wave-private: wave v ranks its 1024 keys round by round against its own
+// LDS digit counters (ballots give the rank among equal digits of one round),
+// and the waves' counters are prefix-summed once per tile.
+
+constexpr int EV_MAXD = 10;
+constexpr uint32_t EV_BINS = 1u << EV_MAXD;
+constexpr int EV_WKEYS = RS_TILE / (NT / 64);  // keys per wave and tile
+
+struct EvSrc {
+    const uint32_t* segid;
+    const uint32_t* start;
+    const uint32_t* end;
+    uint32_t cb;  // coordinate key bits; the segment sits above them
+};
+
+__device__ inline uint64_t event_key(const EvSrc& s, size_t e) {
+    const size_t i = e >> 1;
+    const uint64_t seg = (uint64_t)s.segid[i] << s.cb;
+    return seg | ((e & 1) ? 2ull * s.end[i] : 2ull * s.start[i] + 1ull);
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(NT) void ev_upsweep(EvSrc src, const uint64_t* keys, size_t m,
+                                                  int shift, uint32_t nbits, uint32_t* hist,
+                                                  uint32_t nblocks) {
+    __shared__ uint32_t h[EV_BINS];
+    const uint32_t bins = 1u << nbits;
+    for (uint32_t d = threadIdx.x; d < bins; d += NT) h[d] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * RS_TILE + threadIdx.x;
+    uint32_t dig[RS_ITEMS];  // all loads issued before the first LDS atomic
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) {
+        const size_t idx = base + (size_t)i * NT;
+        uint64_t k = 0;
+        if (idx < m) k = FIRST ? event_key(src, idx) : keys[idx];
+        dig[i] = idx < m ? (uint32_t)(k >> shift) & (bins - 1) : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++)
+        if (dig[i] != 0xFFFFFFFFu) atomicAdd(&h[dig[i]], 1u);
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < bins; d += NT) hist[(size_t)d * nblocks + blockIdx.x] = h[d];
+}
+
+template <bool FIRST, bool LAST>
+__global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin, uint64_t* kout,
+                                                  size_t m, int shift, uint32_t nbits,
+                                                  const uint32_t* offs, uint32_t nblocks,
+                                                  uint32_t* key32, uint32_t* rs, int64_t* w) {
+    __shared__ uint32_t cnt[NT / 64][EV_BINS];  // per wave digit counts, then tile-local starts
+    __shared__ uint32_t lbase[EV_BINS];          // first tile-local slot of each digit
+    __shared__ uint32_t gbase[EV_BINS];          // first output position of each digit
+    __shared__ uint64_t stage[RS_TILE];          // the tile in digit order
+    __shared__ uint32_t lsum[NT / 64 + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t bins = 1u << nbits;
+    for (uint32_t d = tid; d < bins; d += NT) {
+#pragma unroll
+        for (int v = 0; v < NT / 64; v++) cnt[v][d] = 0;
+    }
+    __syncthreads();
+    const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const size_t tile0 = (size_t)blockIdx.x * RS_TILE;
+    const size_t w0 = tile0 + (size_t)wv * EV_WKEYS;
+    uint64_t key[RS_ITEMS];
+    uint32_t loc[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {  // all loads issued before the ranking rounds
+        const size_t idx = w0 + (size_t)r * 64 + lane;
+        key[r] = 0;
+        if (idx < m) key[r] = FIRST ? event_key(src, idx) : kin[idx];
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const size_t idx = w0 + (size_t)r * 64 + lane;
+        const bool valid = idx < m;
+        const uint64_t k = key[r];
+        const uint32_t d = (uint32_t)(k >> shift) & (bins - 1);
+        uint64_t peers = __ballot(valid);
+        for (uint32_t b = 0; b < nbits; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        // this wave's count of digit d so far, then the group's leader adds
+        // the round's group size (same wave: the read is ordered before it)
+        const uint32_t pre = valid ? cnt[wv][d] : 0u;
+        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && rank == 0) cnt[wv][d] = pre + (uint32_t)__popcll(peers);
+        loc[r] = pre + rank;
+    }
+    __syncthreads();
+    // tile-local layout: digits in order, within a digit wave 0's keys first
+    const uint32_t dpt = bins > NT ? bins / NT : 1u;  // digits per thread (bins is a power of 2)
+    const uint32_t d0 = tid * dpt;
+    uint32_t mine = 0;
+    for (uint32_t j = 0; j < dpt; j++)
+        if (d0 + j < bins) {
+#pragma unroll
+            for (int v = 0; v < NT / 64; v++) mine += cnt[v][d0 + j];
+        }
+    uint32_t ttot;
+    uint32_t run = block_exclusive_scan<uint32_t>(mine, lsum, ttot);
+    for (uint32_t j = 0; j < dpt; j++) {
+        const uint32_t d = d0 + j;
+        if (d < bins) {
+            lbase[d] = run;
+            gbase[d] = offs[(size_t)d * nblocks + blockIdx.x];
+#pragma unroll
+            for (int v = 0; v < NT / 64; v++) {
+                const uint32_t t = cnt[v][d];
+                cnt[v][d] = run;
+                run += t;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const size_t idx = w0 + (size_t)r * 64 + lane;
+        if (idx < m) {
+            const uint32_t d = (uint32_t)(key[r] >> shift) & (bins - 1);
+            stage[cnt[wv][d] + loc[r]] = key[r];
+        }
+    }
+    __syncthreads();
+    // write out in tile order: runs of one digit go to consecutive positions
+    const uint32_t nt = (uint32_t)(m - tile0 < (size_t)RS_TILE ? m - tile0 : (size_t)RS_TILE);
+    const uint64_t cmask = (1ull << src.cb) - 1ull;
+    for (uint32_t j = tid; j < nt; j += NT) {
+        const uint64_t k = stage[j];
+        const uint32_t d = (uint32_t)(k >> shift) & (bins - 1);
+        const uint32_t pos = gbase[d] + (j - lbase[d]);
+        if (LAST) {
+            const uint32_t lo = (uint32_t)(k & cmask);
+            const int64_t c = (int64_t)(lo >> 1);
+            key32[pos] = lo;
+            rs[pos] = lo & 1u;
+            w[pos] = (lo & 1u) ? c : -c;
+        } else {
+            kout[pos] = k;
+        }
+    }
+    if (LAST && blockIdx.x == 0 && tid == 0) key32[m] = 0xFFFFFFFFu;
+}
+
+// Events of an empty batch: the sentinel key and zero totals.
+__global__ void ev_empty(uint32_t* key32, uint32_t* rs, int64_t* w) {
+    if (threadIdx.x == 0) {
+        key32[0] = 0xFFFFFFFFu;
+        rs[0] = 0;
+        w[0] = 0;
+    }
+}
+
+// Joint exclusive scan of rs (u32) and w (i64): tile sums, then the tiles
+// with their bases (reduced in the apply kernel for up to 2048 tiles).
+__global__ __launch_bounds__(NT) void ev_scan_sums(const uint32_t* rs, const int64_t* w, size_t n,
+                                                    uint32_t* srs, int64_t* sw) {
+    __shared__ uint32_t l32[NT / 64 + 1];
+    __shared__ int64_t l64[NT / 64 + 1];
+    const size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_ITEMS;
+    uint32_t va[SC_ITEMS];
+    int64_t vb[SC_ITEMS];
+    load_run<uint32_t>(rs, base, n, va);
+    load_run<int64_t>(w, base, n, vb);
+    uint32_t a = 0;
+    int64_t b = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        a += va[k];
+        b += vb[k];
+    }
+    uint32_t ta;
+    int64_t tb;
+    (void)block_exclusive_scan<uint32_t>(a, l32, ta);
+    (void)block_exclusive_scan<int64_t>(b, l64, tb);
+    if (threadIdx.x == 0) {
+        srs[blockIdx.x] = ta;
+        sw[blockIdx.x] = tb;
+    }
+}
+
+template <bool PRE>
+__global__ __launch_bounds__(NT) void ev_scan_apply(uint32_t* rs, int64_t* w, size_t n,
+                                                     const uint32_t* srs, const int64_t* sw,
+                                                     size_t nb) {
+    __shared__ uint32_t l32[NT / 64 + 1];
+    __shared__ int64_t l64[NT / 64 + 1];
+    uint32_t pa;
+    int64_t pb;
+    if (PRE) {
+        uint32_t a = 0;
+        int64_t b = 0;
+        for (size_t i = threadIdx.x; i < blockIdx.x; i += NT) {
+            a += srs[i];
+            b += sw[i];
+        }
+        (void)block_exclusive_scan<uint32_t>(a, l32, pa);
+        (void)block_exclusive_scan<int64_t>(b, l64, pb);
+    } else {
+        pa = srs[blockIdx.x];
+        pb = sw[blockIdx.x];
+    }
+    const size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_ITEMS;
+    uint32_t va[SC_ITEMS];
+    int64_t vb[SC_ITEMS];
+    load_run<uint32_t>(rs, base, n, va);
+    load_run<int64_t>(w, base, n, vb);
+    uint32_t a = 0;
+    int64_t b = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        a += va[k];
+        b += vb[k];
+    }
+    uint32_t ta;
+    int64_t tb;
+    uint32_t ea = block_exclusive_scan<uint32_t>(a, l32, ta) + pa;
+    int64_t eb = block_exclusive_scan<int64_t>(b, l64, tb) + pb;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        const uint32_t x = va[k];
+        const int64_t y = vb[k];
+        va[k] = ea;
+        vb[k] = eb;
+        ea += x;
+        eb += y;
+    }
+    store_run<uint32_t>(rs, base, n, va);
+    store_run<int64_t>(w, base, n, vb);
+    if (blockIdx.x + 1 == nb && threadIdx.x == 0) {
+        rs[n] = pa + ta;
+        w[n] = pb + tb;
+    }
+}
+
 }  // namespace
+
+void sort_events(const uint32_t* segid, const uint32_t* start, const uint32_t* end, size_t n,
+                 uint32_t seg_bits, uint32_t coord_bits, uint32_t* key32, uint32_t* rs,
+                 int64_t* w, uint64_t* keys, SortScratch& s, hipStream_t st) {
+    const size_t m = 2 * n;
+    if (m == 0) {
+        hipLaunchKernelGGL(ev_empty, dim3(1), dim3(64), 0, st, key32, rs, w);
+        CHK_HIP(hipGetLastError());
+        return;
+    }
+    CHK_REQUIRE(m < (1ull << 32), CHK_ERR_INVALID, "event sort supports fewer than 2^32 events");
+    CHK_REQUIRE(coord_bits >= 1 && coord_bits <= 32 && seg_bits <= 31, CHK_ERR_INTERNAL,
+                "event key widths out of range");
+    const EvSrc src{segid, start, end, coord_bits};
+    const uint32_t bits = coord_bits + seg_bits;
+    const uint32_t P = (bits + EV_MAXD - 1) / EV_MAXD;
+    const uint32_t nblocks = (uint32_t)((m + RS_TILE - 1) / RS_TILE);
+    if (P > 1) s.keys_alt.reserve(m);
+    const size_t hn = (size_t)nblocks << (bits / P + (bits % P ? 1 : 0));
+    s.hist.reserve(hn + 1);
+    uint64_t* buf[2] = {keys, s.keys_alt.p};
+    int shift = 0;
+    for (uint32_t p = 0; p < P; p++) {
+        const uint32_t nb = bits / P + (p < bits % P ? 1u : 0u);  // digit width, <= EV_MAXD
+        const bool first = p == 0, last = p + 1 == P;
+        const uint64_t* kin = first ? nullptr : buf[(p - 1) & 1];
+        uint64_t* kout = last ? nullptr : buf[p & 1];
+        if (first)
+            hipLaunchKernelGGL(ev_upsweep<true>, dim3(nblocks), dim3(NT), 0, st, src, kin, m, shift, nb,
+                               s.hist.p, nblocks);
+        else
+            hipLaunchKernelGGL(ev_upsweep<false>, dim3(nblocks), dim3(NT), 0, st, src, kin, m, shift, nb,
+                               s.hist.p, nblocks);
+        exclusive_scan_u32(s.hist.p, (size_t)nblocks << nb, s, st);
+        auto scatter = first ? (last ? ev_scatter<true, true> : ev_scatter<true, false>)
+                             : (last ? ev_scatter<false, true> : ev_scatter<false, false>);
+        hipLaunchKernelGGL(scatter, dim3(nblocks), dim3(NT), 0, st, src, kin, kout, m, shift, nb,
+                           s.hist.p, nblocks, key32, rs, w);
+        CHK_HIP(hipGetLastError());
+        shift += (int)nb;
+    }
+    // exclusive prefix sums of rs and w, totals at [m]
+    const size_t nbs = (m + SC_TILE - 1) / SC_TILE;
+    s.sums.reserve(nbs + 1);
+    s.sums64.reserve(nbs + 1);
+    hipLaunchKernelGGL(ev_scan_sums, dim3((unsigned)nbs), dim3(NT), 0, st, rs, w, m, s.sums.p, s.sums64.p);
+    if (nbs <= 2048) {
+        hipLaunchKernelGGL(ev_scan_apply<true>, dim3((unsigned)nbs), dim3(NT), 0, st, rs, w, m, s.sums.p,
+                           s.sums64.p, nbs);
+    } else {
+        hipLaunchKernelGGL(scan_sums_single<uint32_t>, dim3(1), dim3(NT), 0, st, s.sums.p, nbs);
+        hipLaunchKernelGGL(scan_sums_single<int64_t>, dim3(1), dim3(NT), 0, st, s.sums64.p, nbs);
+        hipLaunchKernelGGL(ev_scan_apply<false>, dim3((unsigned)nbs), dim3(NT), 0, st, rs, w, m, s.sums.p,
+                           s.sums64.p, nbs);
+    }
+    CHK_HIP(hipGetLastError());
+}
 
 void exclusive_scan_u32(uint32_t* data, size_t n, SortScratch& s, hipStream_t st) {
     s.sums.reserve((n + SC_TILE - 1) / SC_TILE + 1);

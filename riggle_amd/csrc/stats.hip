@@ -278,7 +278,7 @@ void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, doubl
                 q.g_shard.p, q.g_ss.p, q.g_ss2.p, q.g_cnt.p, ix.mom.d_nbins.p, ix.mom.d_slot_off.p,
                 ix.mom.d_slot_len.p, ix.mom.d_has_entry.p, ix.mom.mean.p, ix.mom.var.p, d_p, d_llr,
                 d_flags};
-    hipEvent_t t0 = ix.prof.begin(st);
+    hipEvent_t t0 = ix.prof.begin("pvalues", st);
     hipLaunchKernelGGL(pvalue_kernel, dim3((cells + 3) / 4), dim3(256), 0, st, a);
     CHK_HIP(hipGetLastError());
     ix.prof.end("pvalues", t0, st);

@@ -619,10 +619,10 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
     zero_u64(d_bp, cells, st);
     if (!q.jobs_host.empty()) {
         zero_u64(reinterpret_cast<uint64_t*>(q.queue.p), 1, st);
-        hipEvent_t t0 = ix.prof.begin(st);
+        hipEvent_t t0 = ix.prof.begin("qprep", st);
         query_events(q, st);
         ix.prof.end("qprep", t0, st);
-        t0 = ix.prof.begin(st);
+        t0 = ix.prof.begin("partition", st);
         hipLaunchKernelGGL(partition_kernel, dim3((q.nbounds + NT - 1) / NT), dim3(NT), 0, st,
                            q.jobs.p, (uint32_t)q.jobs_host.size(), q.nbounds, q.tile, ix.s_coord.p,
                            ix.e_coord.p, q.key32.p, q.part.p);
@@ -633,7 +633,7 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
                        q.w.p, q.queue.p,
                        reinterpret_cast<unsigned long long*>(d_counts),
                        reinterpret_cast<unsigned long long*>(d_bp)};
-        t0 = ix.prof.begin(st);
+        t0 = ix.prof.begin("sweep", st);
         switch (q.items) {
             case 4: launch_sweep<4>(args, q.grid, st); break;
             case 8: launch_sweep<8>(args, q.grid, st); break;
@@ -648,7 +648,7 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
                       reinterpret_cast<unsigned long long*>(d_counts),
                       reinterpret_cast<unsigned long long*>(d_bp)};
         const uint32_t ni = (uint32_t)q.items_host.size();
-        hipEvent_t t0 = ix.prof.begin(st);
+        hipEvent_t t0 = ix.prof.begin("sparse", st);
         hipLaunchKernelGGL(sparse_kernel, dim3(ni), dim3(NT), 0, st, sa);
         CHK_HIP(hipGetLastError());
         ix.prof.end("sparse", t0, st);

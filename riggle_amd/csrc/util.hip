@@ -17,7 +17,7 @@ void calibrate_copy(Index& ix, uint64_t bytes) {
     a.reserve(n);
     b.reserve(n);
     CHK_HIP(hipMemsetAsync(a.p, 1, n * sizeof(uint4), ix.stream));
-    hipEvent_t t0 = ix.prof.begin(ix.stream);
+    hipEvent_t t0 = ix.prof.begin("calib_copy", ix.stream);
     hipLaunchKernelGGL(calib_copy_kernel, dim3(ix.ncu * 8), dim3(256), 0, ix.stream, a.p, b.p, n);
     CHK_HIP(hipGetLastError());
     ix.prof.end("calib_copy", t0, ix.stream);
