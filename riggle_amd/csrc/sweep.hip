@@ -42,7 +42,8 @@ void query_events(Query& q, hipStream_t st);
 namespace {
 
 constexpr int NT = 256;
-constexpr uint32_t QS = 512;    // events staged per tile on the fast path (nq + 1 <= QS)
+constexpr uint32_t QS = 256;    // events staged per tile on the fast path (nq + 1 <= QS)
+constexpr int QH = QS / NT;     // staged events per thread
 constexpr uint32_t MAXC = 64;   // tiles per work item at most
 constexpr uint32_t SPI = 256;   // query intervals per sparse work item at most (default)
 
@@ -138,8 +139,8 @@ template <int ITEMS>
 struct Stage {
     int32_t x[ITEMS];
     uint32_t f[ITEMS];
-    uint32_t key[2], rs[2];
-    int64_t w[2];
+    uint32_t key[QH], rs[QH];
+    int64_t w[QH];
 };
 
 // 8-item tiles fit 96 VGPRs (5 waves per SIMD) without spills; 16-item tiles
@@ -149,8 +150,11 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
     constexpr uint32_t IPW = 64u * ITEMS;  // DB elements per wave per tile
     constexpr uint32_t TILE = NT * ITEMS;
     __shared__ unsigned long long hc[FG], hb[FG];
-    __shared__ uint32_t lkey[QS], lrs[QS];
-    __shared__ int64_t lw[QS];
+    // staged query events, double-buffered (register stage A uses buffer 0,
+    // B buffer 1): a tile's events are written while the other buffer may
+    // still be read, so one barrier per tile orders the staging
+    __shared__ uint32_t lkey2[2][QS], lrs2[2][QS];
+    __shared__ int64_t lw2[2][QS];
     __shared__ TMeta tm[MAXC];
     __shared__ Job lj[MAXC + 1];
     __shared__ uint32_t s_chunk;
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
         const auto rr = make_rsrc(a.qrs + m.qb, nst * 4u);
         const auto rw = make_rsrc(a.qw + m.qb, nst * 8u);
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < QH; h++) {
             const uint32_t i = tid + h * NT;
             s.key[h] = __builtin_amdgcn_raw_buffer_load_b32(rk, i * 4u, 0, 0);
             s.rs[h] = __builtin_amdgcn_raw_buffer_load_b32(rr, i * 4u, 0, 0);
@@ -215,14 +219,16 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
         }
     };
 
-    auto stage_store = [&](const Stage<ITEMS>& s, const Meta& m) {
-        if (m.cell != cur_cell) {
+    auto stage_store = [&](const Stage<ITEMS>& s, const Meta& m, uint32_t* lkey, uint32_t* lrs,
+                           int64_t* lw) {
+        if (m.cell != cur_cell) {  // uniform over the block
+            lds_barrier();         // every wave's atomics into the histogram are done
             flush();
             cur_cell = m.cell;
             cur_nf = m.nf;
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < QH; h++) {
             const uint32_t i = tid + h * NT;
             if (i <= m.nq) {  // i < QS always
                 lkey[i] = i < m.nq ? s.key[h] : 0xFFFFFFFFu;
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
     };
 
     // rank of key dk among the staged events: two-level ballot search
-    auto rank_lds = [&](uint32_t dk, uint32_t nq) {
+    auto rank_lds = [&](uint32_t dk, uint32_t nq, const uint32_t* lkey) {
         const uint32_t si = lane * 8u;
         const uint32_t sv = si < nq ? lkey[si] : 0xFFFFFFFFu;
         const uint32_t c1 = popc(__ballot(sv < dk));
@@ -257,7 +263,8 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
         return base + popc(__ballot(lv < dk));
     };
 
-    auto compute = [&](const Stage<ITEMS>& s, const Meta& m) {
+    auto compute = [&](const Stage<ITEMS>& s, const Meta& m, const uint32_t* lkey, const uint32_t* lrs,
+                       const int64_t* lw) {
         const uint32_t nq = m.nq, nd = m.nd;
         const uint32_t wbase = wv * IPW;
         if (wbase >= nd) return;
@@ -265,7 +272,7 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
             uint32_t jlo = 0, kv = 0, rv = 0;
             int64_t wvv = 0;
             auto window = [&](uint32_t dk) {
-                jlo = ufl(rank_lds(dk, nq));
+                jlo = ufl(rank_lds(dk, nq, lkey));
                 const uint32_t wi = jlo + lane;
                 kv = wi < nq ? lkey[wi] : 0xFFFFFFFFu;
                 rv = wi <= nq ? lrs[wi] : 0u;
@@ -292,18 +299,26 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
                         const uint32_t dk = db_key(s.x[k]);
                         const uint32_t dkf = ufl(dk), dkl = rdl(dk, nv - 1u);
                         const uint32_t ea = popc(__ballot(kv < dkf)), eb = popc(__ballot(kv < dkl));
-                        uint32_t c = ea;
-                        uint32_t rsv = rdl(rv, ea);
-                        int64_t wq = rdl64(wvv, ea);
-                        for (uint32_t t = ea; t < eb; t++) {
-                            const bool past = rdl(kv, t) < dk;
-                            c += past ? 1u : 0u;
-                            const uint32_t r1 = rdl(rv, t + 1);
-                            const int64_t w1 = rdl64(wvv, t + 1);
-                            rsv = past ? r1 : rsv;
-                            wq = past ? w1 : wq;
+                        const uint32_t pa = m.j0 + jlo + ea;
+                        const uint32_t rsa = rdl(rv, ea);
+                        const int64_t wa = rdl64(wvv, ea);
+                        if (ea == eb) {
+                            // no event inside the row: every lane has rank pa,
+                            // so the depth and count terms are scalar
+                            if (lane < nv) add(m, s.x[k], s.f[k], pa, rsa, wa);
+                        } else {
+                            uint32_t c = 0, rsv = rsa;
+                            int64_t wq = wa;
+                            for (uint32_t t = ea; t < eb; t++) {
+                                const bool past = rdl(kv, t) < dk;
+                                c += past ? 1u : 0u;
+                                const uint32_t r1 = rdl(rv, t + 1);
+                                const int64_t w1 = rdl64(wvv, t + 1);
+                                rsv = past ? r1 : rsv;
+                                wq = past ? w1 : wq;
+                            }
+                            if (lane < nv) add(m, s.x[k], s.f[k], pa + c, rsv, wq);
                         }
-                        if (lane < nv) add(m, s.x[k], s.f[k], m.j0 + jlo + c, rsv, wq);
                     }
                 }
             } else {
@@ -388,17 +403,15 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
         Meta mB = meta(min(1u, nt - 1));
         for (uint32_t k = 0;; k += 2) {
             stage_load(mB, B);
-            stage_store(A, mA);
+            stage_store(A, mA, lkey2[0], lrs2[0], lw2[0]);
             lds_barrier();
-            compute(A, mA);
-            lds_barrier();
+            compute(A, mA, lkey2[0], lrs2[0], lw2[0]);
             if (k + 1 >= nt) break;
             mA = meta(min(k + 2, nt - 1));
             stage_load(mA, A);
-            stage_store(B, mB);
+            stage_store(B, mB, lkey2[1], lrs2[1], lw2[1]);
             lds_barrier();
-            compute(B, mB);
-            lds_barrier();
+            compute(B, mB, lkey2[1], lrs2[1], lw2[1]);
             if (k + 2 >= nt) break;
             mB = meta(min(k + 3, nt - 1));
         }
