@@ -55,6 +55,9 @@ struct Profiler {
 // files per block, so the DB streams are split by (shard, file group) and
 // carry group-local 16-bit file ids.
 constexpr uint32_t FG = 1024;
+
+// Stride of the stream samples that the merge-path split search starts from.
+constexpr uint32_t PSTRIDE = 64;
 // Size classes of the layered start-sorted table (src/io/meta.rs:33-81,
 // LayerConfig::default_genomic: min_size 128, growth factor 4).
 constexpr uint32_t LAYER_MIN = 128, LAYER_GF = 4;
@@ -115,6 +118,7 @@ struct Index {
     std::vector<uint64_t> shard_n;    // intervals per shard
     std::vector<uint64_t> str_off;    // [nshards * ngroups + 1] stream segment offsets
     DevBuf<int32_t> s_coord, e_coord;
+    DevBuf<int32_t> s_samp, e_samp;   // every PSTRIDE-th stream coordinate (coarse split search)
     DevBuf<uint16_t> s_lfid, e_lfid;
     uint32_t nlayers = 0;
     uint64_t span = 0;                // sum over shards of the largest start
@@ -130,7 +134,8 @@ struct Index {
         if (stream) (void)hipStreamDestroy(stream);
     }
     uint64_t device_bytes() const {
-        return s_coord.bytes() + e_coord.bytes() + s_lfid.bytes() + e_lfid.bytes() +
+        return s_coord.bytes() + e_coord.bytes() + s_samp.bytes() + e_samp.bytes() +
+               s_lfid.bytes() + e_lfid.bytes() +
                l_start.bytes() + l_end.bytes() + l_fid.bytes() + jt.bytes() + d_lseg.bytes();
     }
 };
@@ -246,6 +251,9 @@ void query_plan(Query& q);
 void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st);
 // Fill n u64 with zero from a kernel (graph-safe clearing).
 void zero_u64(uint64_t* p, size_t n, hipStream_t st);
+// zero fills of three buffers (a kernel, not a memset: sequences may be graph-captured)
+void zero3_u64(uint64_t* p0, size_t n0, uint64_t* p1, size_t n1, uint64_t* p2, size_t n2,
+               hipStream_t st);
 // Mean time of `reps` sweeps (after one warm-up) with HIP events.
 double time_sweeps(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps);
 // Coordinate-descent autotuner over the sweep knobs for this batch; leaves the

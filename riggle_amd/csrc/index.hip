@@ -104,13 +104,38 @@ __global__ void build_jump(const LayerSeg* segs, uint32_t nseg, const int32_t* l
     }
 }
 
+// samp[g] = coord[g * PSTRIDE]
+__global__ void sample_stream(const int32_t* coord, size_t ns, int32_t* samp) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ns) samp[g] = coord[g * PSTRIDE];
+}
+
 __global__ void fill_u64(uint64_t* p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0ull;
 }
 
+// Three zero fills in one launch.
+__global__ void fill3_u64(uint64_t* p0, size_t n0, uint64_t* p1, size_t n1, uint64_t* p2, size_t n2) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n0 + n1 + n2;
+         i += (size_t)gridDim.x * blockDim.x) {
+        if (i < n0) p0[i] = 0ull;
+        else if (i < n0 + n1) p1[i - n0] = 0ull;
+        else p2[i - n0 - n1] = 0ull;
+    }
+}
+
 }  // namespace
+
+void zero3_u64(uint64_t* p0, size_t n0, uint64_t* p1, size_t n1, uint64_t* p2, size_t n2,
+               hipStream_t st) {
+    const size_t n = n0 + n1 + n2;
+    if (!n) return;
+    const unsigned g = (unsigned)std::min<size_t>(nblk(n), 4096);
+    hipLaunchKernelGGL(fill3_u64, dim3(g), dim3(NT), 0, st, p0, n0, p1, n1, p2, n2);
+    CHK_HIP(hipGetLastError());
+}
 
 void zero_u64(uint64_t* p, size_t n, hipStream_t st) {
     if (!n) return;
@@ -223,6 +248,12 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
         hipLaunchKernelGGL(split_stream_keys, dim3(nblk(n)), dim3(NT), 0, st, keys.p, vals.p,
                            (size_t)n, pass == 0 ? ix.s_coord.p : ix.e_coord.p,
                            pass == 0 ? ix.s_lfid.p : ix.e_lfid.p);
+        const size_t ns = (n + PSTRIDE - 1) / PSTRIDE;
+        DevBuf<int32_t>& samp = pass == 0 ? ix.s_samp : ix.e_samp;
+        samp.reserve(ns + 1);
+        if (ns)
+            hipLaunchKernelGGL(sample_stream, dim3(nblk(ns)), dim3(NT), 0, st,
+                               pass == 0 ? ix.s_coord.p : ix.e_coord.p, ns, samp.p);
         CHK_HIP(hipGetLastError());
     }
     // layered table + jump tables

@@ -76,8 +76,11 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t by
 
 // Merge-path split of every tile boundary: part[b] = # DB elements among the
 // first d merged items of its job (DB element before event iff dkey <= qkey).
+// The search first narrows the split to PSTRIDE elements on the stream
+// samples (small and cache-resident), then finishes on the stream itself.
 __global__ void partition_kernel(const Job* jobs, uint32_t njobs, uint32_t nbound, uint32_t tile,
                                  const int32_t* s_coord, const int32_t* e_coord,
+                                 const int32_t* s_samp, const int32_t* e_samp,
                                  const uint32_t* qkeys, uint32_t* part) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbound) return;
@@ -92,6 +95,19 @@ __global__ void partition_kernel(const Job* jobs, uint32_t njobs, uint32_t nboun
     uint64_t total = (uint64_t)J.n_db + J.m;
     uint64_t d = std::min<uint64_t>((uint64_t)(b - J.part0) * tile, total);
     uint64_t a = d > J.m ? d - J.m : 0, z = std::min<uint64_t>(d, J.n_db);
+    if (z - a > 2 * PSTRIDE) {
+        // samples at stream positions p = g * PSTRIDE - db_off inside [a, z)
+        const int32_t* S = J.is_end ? e_samp : s_samp;
+        const uint64_t g0 = (J.db_off + a + PSTRIDE - 1) / PSTRIDE, g1 = (J.db_off + z) / PSTRIDE;
+        uint64_t lo = g0, hi = g1;  // first sample where the split is at or before p
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) >> 1;
+            const uint64_t p = mid * PSTRIDE - J.db_off;
+            if (db_key(S[mid]) <= Q[d - 1 - p]) lo = mid + 1; else hi = mid;
+        }
+        if (lo > g0) a = (lo - 1) * PSTRIDE - J.db_off + 1;
+        if (lo < g1) z = lo * PSTRIDE - J.db_off;
+    }
     while (a < z) {
         uint64_t mid = (a + z) >> 1;
         if (db_key(D[mid]) <= Q[d - 1 - mid]) a = mid + 1; else z = mid;
@@ -628,17 +644,16 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
     CHK_REQUIRE(q.plan_version == ix.tune_version, CHK_ERR_INTERNAL, "sweep plan is stale");
     const uint32_t F = ix.nfiles;
     const size_t cells = (size_t)q.nfiles * F;
-    zero_u64(d_counts, cells, st);
-    zero_u64(d_bp, cells, st);
+    zero3_u64(d_counts, cells, d_bp, cells, reinterpret_cast<uint64_t*>(q.queue.p),
+              q.jobs_host.empty() ? 0 : 1, st);
     if (!q.jobs_host.empty()) {
-        zero_u64(reinterpret_cast<uint64_t*>(q.queue.p), 1, st);
         hipEvent_t t0 = ix.prof.begin("qprep", st);
         query_events(q, st);
         ix.prof.end("qprep", t0, st);
         t0 = ix.prof.begin("partition", st);
         hipLaunchKernelGGL(partition_kernel, dim3((q.nbounds + NT - 1) / NT), dim3(NT), 0, st,
                            q.jobs.p, (uint32_t)q.jobs_host.size(), q.nbounds, q.tile, ix.s_coord.p,
-                           ix.e_coord.p, q.key32.p, q.part.p);
+                           ix.e_coord.p, ix.s_samp.p, ix.e_samp.p, q.key32.p, q.part.p);
         CHK_HIP(hipGetLastError());
         ix.prof.end("partition", t0, st);
         SweepArgs args{q.jobs.p, q.chunk_job.p, q.part.p, q.ntiles, q.nchunks, q.chunk, F,
