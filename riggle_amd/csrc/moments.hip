@@ -418,6 +418,9 @@ __global__ __launch_bounds__(NT) void coverage_rows(int32_t* D, uint32_t ldg, ui
 
 // ------------------------------------------------------------ evaluation
 
+constexpr uint32_t EK = 16;       // consecutive positions per thread in the evaluation scan
+constexpr uint32_t CH = NT * EK;  // positions per chunk
+
 struct EvalArgs {
     const int32_t* G;
     uint32_t ldg, n, nl;
@@ -428,6 +431,7 @@ struct EvalArgs {
     const int32_t* pmapM;    // [n+2] position -> L index sampled at i = n-L+1, or -1
     const uint32_t* ls;      // [nl] stored block sizes
     const int64_t* out_off;  // [B] first table entry of the row's slot
+    const uint8_t* runflag;  // [(n + 1) / EK + 1] 1 when run r holds a sampled position
     double* mean;
     double* var;
     // per row scratch of samples: [B][nl] each
@@ -490,6 +494,8 @@ __device__ Terms block_excl_scan_terms(Terms v, Terms* lds, Terms& total) {
 __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
     __shared__ Terms lds[NT / 64 + 1];
     __shared__ int64_t ldsi[NT / 64 + 1];
+    __shared__ int32_t sG[CH + CH / EK];  // chunk of G, one pad word after every run
+    __shared__ double sA[CH + CH / EK];
     __shared__ uint32_t sh_exact;
     const uint32_t b = blockIdx.x, n = a.n;
     const int32_t* G = a.G + (size_t)b * a.ldg;
@@ -524,33 +530,42 @@ __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
     const bool exact = sh_exact != 0;
     Terms carry = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
     int64_t pcarry = 0;
-    // Terms of position i (0..n) given P = P[i] = sum_{j<i} G[j].
-    auto term = [&](uint32_t i, int64_t P) {
+    // Terms of position i (0..n) given P = P[i] = sum_{j<i} G[j] and av = A[i]
+    // (rounded when exact; 0 outside 1 <= i < n).
+    auto term = [&](uint32_t i, int64_t P, double av) {
         Terms t = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
         t.pp = P;
         const double ph = (double)(P - c * (int64_t)i);
         const double qm = (double)(PHn - (P - c * (int64_t)i));
         if (i < n) t.q = dd_from_prod(qm, qm);
         if (i >= 1 && i < n) {
-            double av = A[i];
-            if (exact) av = rint(av);
             t.c1 = {av, 0.0};
             t.c2 = dd_from_prod(av, (double)i);
             t.e1 = dd_from_prod(ph, ph);
         }
         return t;
     };
-    // positions 0..n (n+1 values), EK consecutive positions per thread: a
-    // serial pass over the thread's run, one block scan of the run totals,
-    // then a second serial pass that emits the exclusive prefixes sampled at
-    // the stored block sizes
-    constexpr uint32_t EK = 16;
-    for (uint64_t base = 0; base <= n; base += (uint64_t)NT * EK) {
+    // positions 0..n (n+1 values) in chunks of CH: the chunk's G and A are
+    // staged in LDS with coalesced loads (padded so that each thread's run of
+    // EK consecutive positions is read without bank conflicts); then a serial
+    // pass over the thread's run, one block scan of the run totals, and a
+    // second serial pass, only over runs that hold a sampled position, that
+    // emits the exclusive prefixes at the stored block sizes
+    for (uint64_t base = 0; base <= n; base += CH) {
+        __syncthreads();  // the previous chunk's LDS reads are done
+        for (uint32_t j = threadIdx.x; j < CH; j += NT) {
+            const uint64_t i = base + j;
+            sG[j + j / EK] = i < n ? G[i] : 0;
+            double av = (i >= 1 && i < n) ? A[i] : 0.0;
+            if (exact) av = rint(av);
+            sA[j + j / EK] = av;
+        }
+        __syncthreads();
+        const uint32_t r0 = threadIdx.x * (EK + 1);
         const uint64_t i0 = base + (uint64_t)threadIdx.x * EK;
         int64_t gsum = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < EK; k++)
-            if (i0 + k < n) gsum += (int64_t)G[i0 + k];
+        for (uint32_t k = 0; k < EK; k++) gsum += (int64_t)sG[r0 + k];  // 0 past n
         int64_t gtot;
         const int64_t gex = block_incl_scan<int64_t>(gsum, ldsi, gtot) - gsum;
         Terms loc = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
@@ -558,29 +573,31 @@ __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
         for (uint32_t k = 0; k < EK; k++) {
             const uint64_t i = i0 + k;
             if (i > n) break;
-            loc = terms_add(loc, term((uint32_t)i, P));
-            if (i < n) P += (int64_t)G[i];
+            loc = terms_add(loc, term((uint32_t)i, P, sA[r0 + k]));
+            P += (int64_t)sG[r0 + k];
         }
         Terms tot;
         Terms run = terms_add(block_excl_scan_terms(loc, lds, tot), carry);
-        P = pcarry + gex;
-        const size_t rb = (size_t)b * a.nl;
-        for (uint32_t k = 0; k < EK; k++) {
-            const uint64_t i = i0 + k;
-            if (i > n) break;
-            const int32_t jl = a.pmapL[i], jm = a.pmapM[i];
-            if (jl >= 0) {
-                a.sPP_L[rb + jl] = run.pp;
-                a.sC1[rb + jl] = run.c1;
-                a.sC2[rb + jl] = run.c2;
-                a.sE1[rb + jl] = run.e1;
+        if (i0 <= n && a.runflag[i0 / EK]) {
+            P = pcarry + gex;
+            const size_t rb = (size_t)b * a.nl;
+            for (uint32_t k = 0; k < EK; k++) {
+                const uint64_t i = i0 + k;
+                if (i > n) break;
+                const int32_t jl = a.pmapL[i], jm = a.pmapM[i];
+                if (jl >= 0) {
+                    a.sPP_L[rb + jl] = run.pp;
+                    a.sC1[rb + jl] = run.c1;
+                    a.sC2[rb + jl] = run.c2;
+                    a.sE1[rb + jl] = run.e1;
+                }
+                if (jm >= 0) {
+                    a.sPP_M[rb + jm] = run.pp;
+                    a.sQ[rb + jm] = run.q;
+                }
+                run = terms_add(run, term((uint32_t)i, P, sA[r0 + k]));
+                P += (int64_t)sG[r0 + k];
             }
-            if (jm >= 0) {
-                a.sPP_M[rb + jm] = run.pp;
-                a.sQ[rb + jm] = run.q;
-            }
-            run = terms_add(run, term((uint32_t)i, P));
-            if (i < n) P += (int64_t)G[i];
         }
         carry = terms_add(carry, tot);
         pcarry += gtot;
@@ -651,6 +668,7 @@ void moments_build(Index& ix) {
     twN.reserve(FFT_MAXM);
     CHK_HIP(hipMemcpyAsync(tw.p, twh.data(), TW * sizeof(double2), hipMemcpyHostToDevice, st));
     DevBuf<int32_t> slot_of_fid, pmapL, pmapM, D;
+    DevBuf<uint8_t> runflag;
     DevBuf<int64_t> cen, sumg, out_off, sPP_L, sPP_M;
     DevBuf<dd> sC1, sC2, sE1, sQ;
     DevBuf<uint32_t> d_ls, d_exact;
@@ -693,16 +711,23 @@ void moments_build(Index& ix) {
             CHK_HIP(hipMemcpyAsync(twN.p, twnh.data(), FFT_MAXM * sizeof(double2), hipMemcpyHostToDevice, st));
         }
         // sample maps: position L -> index of L; position n-L+1 -> index of L
+        // and the runs of EK positions that hold either
         std::vector<int32_t> pl(n + 2, -1), pm(n + 2, -1);
+        const size_t nrun = (size_t)n / EK + 1;
+        std::vector<uint8_t> rf(nrun, 0);
         for (uint32_t j = 0; j < nl; j++) {
             pl[ls[j]] = (int32_t)j;
             pm[n - ls[j] + 1] = (int32_t)j;
+            rf[ls[j] / EK] = 1;
+            rf[(n - ls[j] + 1) / EK] = 1;
         }
         pmapL.reserve(n + 2);
         pmapM.reserve(n + 2);
+        runflag.reserve(nrun);
         d_ls.reserve(nl);
         CHK_HIP(hipMemcpyAsync(pmapL.p, pl.data(), (n + 2) * 4, hipMemcpyHostToDevice, st));
         CHK_HIP(hipMemcpyAsync(pmapM.p, pm.data(), (n + 2) * 4, hipMemcpyHostToDevice, st));
+        CHK_HIP(hipMemcpyAsync(runflag.p, rf.data(), nrun, hipMemcpyHostToDevice, st));
         CHK_HIP(hipMemcpyAsync(d_ls.p, ls.data(), nl * 4, hipMemcpyHostToDevice, st));
         // rows per batch: as many as a third of the free HBM holds (FFT
         // workspace, autocorrelation, coverage), so long chromosomes still
@@ -766,7 +791,7 @@ void moments_build(Index& ix) {
             CHK_HIP(hipGetLastError());
             ix.prof.end("moments_fft", t0, st);
             EvalArgs ea{D.p, ldg, n, nl, A.p, cen.p, sumg.p, pmapL.p, pmapM.p, d_ls.p, out_off.p,
-                        M.mean.p, M.var.p, sPP_L.p, sPP_M.p, sC1.p, sC2.p, sE1.p, sQ.p, d_exact.p};
+                        runflag.p, M.mean.p, M.var.p, sPP_L.p, sPP_M.p, sC1.p, sC2.p, sE1.p, sQ.p, d_exact.p};
             t0 = ix.prof.begin("moments_eval", st);
             hipLaunchKernelGGL(eval_rows, dim3(nb), dim3(NT), 0, st, ea);
             CHK_HIP(hipGetLastError());

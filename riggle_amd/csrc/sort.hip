@@ -263,10 +263,13 @@ struct EvSrc {
     uint32_t cb;  // coordinate key bits; the segment sits above them
 };
 
-__device__ inline uint64_t event_key(const EvSrc& s, size_t e) {
-    const size_t i = e >> 1;
+// The first pass reads each interval once and takes its two events in turn:
+// tile t holds the events of intervals [t * RS_TILE / 2, (t + 1) * RS_TILE / 2)
+// (any fixed order of the input is fine for an LSD sort's first pass).
+__device__ inline void interval_keys(const EvSrc& s, size_t i, uint64_t& ks, uint64_t& ke) {
     const uint64_t seg = (uint64_t)s.segid[i] << s.cb;
-    return seg | ((e & 1) ? 2ull * s.end[i] : 2ull * s.start[i] + 1ull);
+    ks = seg | (2ull * s.start[i] + 1ull);
+    ke = seg | (2ull * s.end[i]);
 }
 
 template <bool FIRST>
@@ -279,12 +282,23 @@ __global__ __launch_bounds__(NT) void ev_upsweep(EvSrc src, const uint64_t* keys
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * RS_TILE + threadIdx.x;
     uint32_t dig[RS_ITEMS];  // all loads issued before the first LDS atomic
+    if (FIRST) {
+        const size_t n = m / 2, i0 = (size_t)blockIdx.x * (RS_TILE / 2) + threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; i++) {
-        const size_t idx = base + (size_t)i * NT;
-        uint64_t k = 0;
-        if (idx < m) k = FIRST ? event_key(src, idx) : keys[idx];
-        dig[i] = idx < m ? (uint32_t)(k >> shift) & (bins - 1) : 0xFFFFFFFFu;
+        for (int i = 0; i < RS_ITEMS / 2; i++) {
+            const size_t iv = i0 + (size_t)i * NT;
+            uint64_t ks = 0, ke = 0;
+            if (iv < n) interval_keys(src, iv, ks, ke);
+            dig[2 * i] = iv < n ? (uint32_t)(ks >> shift) & (bins - 1) : 0xFFFFFFFFu;
+            dig[2 * i + 1] = iv < n ? (uint32_t)(ke >> shift) & (bins - 1) : 0xFFFFFFFFu;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < RS_ITEMS; i++) {
+            const size_t idx = base + (size_t)i * NT;
+            const uint64_t k = idx < m ? keys[idx] : 0ull;
+            dig[i] = idx < m ? (uint32_t)(k >> shift) & (bins - 1) : 0xFFFFFFFFu;
+        }
     }
 #pragma unroll
     for (int i = 0; i < RS_ITEMS; i++)
@@ -315,16 +329,28 @@ __global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin,
     const size_t w0 = tile0 + (size_t)wv * EV_WKEYS;
     uint64_t key[RS_ITEMS];
     uint32_t loc[RS_ITEMS];
+    bool val[RS_ITEMS];
+    if (FIRST) {  // rounds 2q, 2q+1: start and end events of interval iv(q)
+        const size_t n = m / 2;
+        const size_t i0 = (size_t)blockIdx.x * (RS_TILE / 2) + (size_t)wv * (EV_WKEYS / 2) + lane;
 #pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++) {  // all loads issued before the ranking rounds
-        const size_t idx = w0 + (size_t)r * 64 + lane;
-        key[r] = 0;
-        if (idx < m) key[r] = FIRST ? event_key(src, idx) : kin[idx];
+        for (int q = 0; q < RS_ITEMS / 2; q++) {
+            const size_t iv = i0 + (size_t)q * 64;
+            key[2 * q] = key[2 * q + 1] = 0;
+            if (iv < n) interval_keys(src, iv, key[2 * q], key[2 * q + 1]);
+            val[2 * q] = val[2 * q + 1] = iv < n;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < RS_ITEMS; r++) {  // all loads issued before the ranking rounds
+            const size_t idx = w0 + (size_t)r * 64 + lane;
+            key[r] = idx < m ? kin[idx] : 0ull;
+            val[r] = idx < m;
+        }
     }
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        const size_t idx = w0 + (size_t)r * 64 + lane;
-        const bool valid = idx < m;
+        const bool valid = val[r];
         const uint64_t k = key[r];
         const uint32_t d = (uint32_t)(k >> shift) & (bins - 1);
         uint64_t peers = __ballot(valid);
@@ -368,8 +394,7 @@ __global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin,
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        const size_t idx = w0 + (size_t)r * 64 + lane;
-        if (idx < m) {
+        if (val[r]) {
             const uint32_t d = (uint32_t)(key[r] >> shift) & (bins - 1);
             stage[cnt[wv][d] + loc[r]] = key[r];
         }
