@@ -32,6 +32,7 @@
 // every overlapping pair.  Its cost follows the query's pairs instead of the
 // DB's size.
 #include <algorithm>
+#include <type_traits>
 
 #include "engine.hpp"
 
@@ -268,6 +269,50 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
         atomicAdd(&hb[f], bv);
     };
 
+    // Contribution of one row at ranks P (per lane) with prefix values (rsv, wq),
+    // for one stream kind (see add).
+    auto add_t = [&](auto end_c, int32_t x, uint32_t f, uint32_t P, uint32_t rsv, int64_t wq) {
+        constexpr bool IS_END = decltype(end_c)::value;
+        const int32_t depth = 2 * (int32_t)rsv - (int32_t)P;
+        const int32_t dep = IS_END ? depth : -depth;
+        const int64_t wn = IS_END ? -wq : wq;
+        const unsigned long long cv =
+            (unsigned long long)(IS_END ? (int64_t)rsv : (int64_t)rsv - (int64_t)P);
+        atomicAdd(&hc[f], cv);
+        atomicAdd(&hb[f], (unsigned long long)((int64_t)x * (int64_t)dep + wn));
+    };
+
+    // A full wave span (ITEMS rows of 64 elements) ranked against the register
+    // window, for one stream kind: no per-row bounds checks, and in a row with
+    // no event inside every lane has the same rank, so its depth, count and W
+    // terms are scalar.
+    auto rows_full = [&](auto end_c, const Stage<ITEMS>& s, uint32_t p0, uint32_t kv, uint32_t rv,
+                         int64_t wvv) {
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t dk = db_key(s.x[k]);
+            const uint32_t ea = popc(__ballot(kv < ufl(dk)));
+            const uint32_t eb = popc(__ballot(kv < rdl(dk, 63u)));
+            const uint32_t rsa = rdl(rv, ea);
+            const int64_t wa = rdl64(wvv, ea);
+            if (ea == eb) {
+                add_t(end_c, s.x[k], s.f[k], p0 + ea, rsa, wa);
+            } else {
+                uint32_t c = 0, rsv = rsa;
+                int64_t wq = wa;
+                for (uint32_t t = ea; t < eb; t++) {
+                    const bool past = rdl(kv, t) < dk;
+                    c += past ? 1u : 0u;
+                    const uint32_t r1 = rdl(rv, t + 1);
+                    const int64_t w1 = rdl64(wvv, t + 1);
+                    rsv = past ? r1 : rsv;
+                    wq = past ? w1 : wq;
+                }
+                add_t(end_c, s.x[k], s.f[k], p0 + ea + c, rsv, wq);
+            }
+        }
+    };
+
     // rank of key dk among the staged events: two-level ballot search
     auto rank_lds = [&](uint32_t dk, uint32_t nq, const uint32_t* lkey) {
         const uint32_t si = lane * 8u;
@@ -303,7 +348,12 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
 #pragma unroll
             for (int k = 0; k < ITEMS; k++)
                 if ((uint32_t)k == kl) dkw = rdl(db_key(s.x[k]), ll);
-            if (popc(__ballot(kv < dkw)) < 64u) {
+            if (popc(__ballot(kv < dkw)) < 64u && nvw == IPW) {
+                if (m.is_end)
+                    rows_full(std::true_type{}, s, m.j0 + jlo, kv, rv, wvv);
+                else
+                    rows_full(std::false_type{}, s, m.j0 + jlo, kv, rv, wvv);
+            } else if (popc(__ballot(kv < dkw)) < 64u) {
                 // Row rank offset c in [ea, eb] and the prefix values at it,
                 // all from the register window: no LDS read in this loop, so
                 // the rows' atomics are never waited for inside the tile.
