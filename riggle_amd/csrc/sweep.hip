@@ -333,7 +333,8 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
             uint32_t jlo = 0, kv = 0, rv = 0;
             int64_t wvv = 0;
             auto window = [&](uint32_t dk) {
-                jlo = ufl(rank_lds(dk, nq, lkey));
+                // with fewer than 64 events the window is all of them
+                jlo = nq < 64u ? 0u : ufl(rank_lds(dk, nq, lkey));
                 const uint32_t wi = jlo + lane;
                 kv = wi < nq ? lkey[wi] : 0xFFFFFFFFu;
                 rv = wi <= nq ? lrs[wi] : 0u;
