@@ -300,8 +300,11 @@ def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
     log(f"serving: {pool} distinct query files ({int(sizes.sum())} intervals) in "
         f"{time.time() - t:.1f}s")
     srv = riggle_amd.Server(ix, slots=3, max_batch_files=16)
-    # each distinct file alone (latency by size), then capacity: closed loop
-    # over the pool, twice (the first pass warms up)
+    # each distinct file alone (latency by size, after one pass over the pool
+    # has grown the slots' buffers), then capacity: closed loop over the pool,
+    # twice (the first pass warms up)
+    for f in files:
+        srv.wait(srv.submit(*f))
     alone = []
     for f in files:
         r = srv.wait(srv.submit(*f))

@@ -175,6 +175,7 @@ struct Query {
     uint64_t n = 0;                   // intervals kept (on shards of the DB)
     std::vector<double> file_len;     // total length per query file
     std::vector<double> file_visits;  // estimated sparse-path visits per query file
+    DevBuf<uint64_t> vis_off, vis;    // their device-side inputs and counters
     std::vector<uint64_t> fs_n;       // [nfiles * nshards] intervals per (file, shard)
     std::vector<uint64_t> file_off;   // canonical layout: first interval of each file
     PinnedBuf<uint32_t> st_shard, st_start, st_end;  // canonical layout, host staging

@@ -445,7 +445,9 @@ void query_visits(Query& q) {
     const uint32_t R = q.nfiles;
     q.file_visits.assign(R, 0.0);
     if (!q.n || ix.n == 0) return;
-    DevBuf<uint64_t> d_off, d_vis;
+    // buffers of the query object: a free here would wait for the whole device
+    DevBuf<uint64_t>& d_off = q.vis_off;
+    DevBuf<uint64_t>& d_vis = q.vis;
     d_off.reserve(R + 1);
     d_vis.reserve(R);
     CHK_HIP(hipMemcpyAsync(d_off.p, q.file_off.data(), (R + 1) * 8, hipMemcpyHostToDevice, q.stream));
