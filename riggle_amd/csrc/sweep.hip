@@ -288,27 +288,39 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
     // terms are scalar.
     auto rows_full = [&](auto end_c, const Stage<ITEMS>& s, uint32_t p0, uint32_t kv, uint32_t rv,
                          int64_t wvv) {
+        // event key 2y + b is below DB key 2x + 1 iff y + b <= x: compare the
+        // window's y + b with the coordinates directly (no key per element);
+        // past the events, 0xFFFFFFFF maps to 2^31, above every coordinate
+        const uint32_t kx = (kv >> 1) + (kv & 1u);
+        uint32_t ce = 0xFFFFFFFFu, rsa = 0;  // prefix values at window event ce
+        int64_t wa = 0;
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
-            const uint32_t dk = db_key(s.x[k]);
-            const uint32_t ea = popc(__ballot(kv < ufl(dk)));
-            const uint32_t eb = popc(__ballot(kv < rdl(dk, 63u)));
-            const uint32_t rsa = rdl(rv, ea);
-            const int64_t wa = rdl64(wvv, ea);
+            const uint32_t x = (uint32_t)s.x[k];
+            const uint32_t ea = popc(__ballot(kx <= ufl(x)));
+            const uint32_t eb = popc(__ballot(kx <= rdl(x, 63u)));
+            if (ea != ce) {  // rows without events keep the previous row's prefix
+                ce = ea;
+                rsa = rdl(rv, ea);
+                wa = rdl64(wvv, ea);
+            }
             if (ea == eb) {
                 add_t(end_c, s.x[k], s.f[k], p0 + ea, rsa, wa);
             } else {
-                uint32_t c = 0, rsv = rsa;
-                int64_t wq = wa;
+                // lanes past event t gain its start count (the key's low bit)
+                // and its signed coordinate; events past a lane past none
+                uint32_t c = 0, drs = 0;
+                int64_t dw = 0;
                 for (uint32_t t = ea; t < eb; t++) {
-                    const bool past = rdl(kv, t) < dk;
+                    const uint32_t kt = rdl(kv, t);
+                    const uint32_t y = kt >> 1, st = kt & 1u;
+                    const bool past = y + st <= x;
+                    const int64_t wi = st ? (int64_t)y : -(int64_t)y;
                     c += past ? 1u : 0u;
-                    const uint32_t r1 = rdl(rv, t + 1);
-                    const int64_t w1 = rdl64(wvv, t + 1);
-                    rsv = past ? r1 : rsv;
-                    wq = past ? w1 : wq;
+                    drs += past ? st : 0u;
+                    dw += past ? wi : 0;
                 }
-                add_t(end_c, s.x[k], s.f[k], p0 + ea + c, rsv, wq);
+                add_t(end_c, s.x[k], s.f[k], p0 + ea + c, rsa + drs, wa + dw);
             }
         }
     };
@@ -346,9 +358,13 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs
             const uint32_t nvw = nd - wbase < IPW ? nd - wbase : IPW;
             const uint32_t kl = (nvw - 1u) >> 6, ll = (nvw - 1u) & 63u;
             uint32_t dkw = 0;
+            if (nvw == IPW) {
+                dkw = rdl(db_key(s.x[ITEMS - 1]), 63u);
+            } else {
 #pragma unroll
-            for (int k = 0; k < ITEMS; k++)
-                if ((uint32_t)k == kl) dkw = rdl(db_key(s.x[k]), ll);
+                for (int k = 0; k < ITEMS; k++)
+                    if ((uint32_t)k == kl) dkw = rdl(db_key(s.x[k]), ll);
+            }
             if (popc(__ballot(kv < dkw)) < 64u && nvw == IPW) {
                 if (m.is_end)
                     rows_full(std::true_type{}, s, m.j0 + jlo, kv, rv, wvv);
