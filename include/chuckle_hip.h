@@ -168,7 +168,7 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
  * (value 0 restores the automatic choice):
  *   "sweep.blocks_per_cu"  resident dense-sweep blocks per CU (persistent grid)
  *   "sweep.chunk"          merge-path tiles per work-queue item (1..64)
- *   "sweep.items"          DB elements per thread per tile: 4, 8 or 16
+ *   "sweep.items"          DB elements per thread per tile: 4, 8, 12 or 16
  *                          (tile = 1024, 2048 or 4096 merged items)
  *   "sweep.sparse_items"   query intervals per sparse work item
  *   "sweep.sparse_max"     largest query file (intervals) swept by the sparse

@@ -78,7 +78,7 @@ double autotune(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps, std::str
     query_plan(q);
     if (!q.jobs_host.empty()) {
         descend("sweep.items", [&] { return (uint64_t)t.items; },
-                [&](uint64_t v) { t.items = (uint32_t)v; }, {4, 8, 16});
+                [&](uint64_t v) { t.items = (uint32_t)v; }, {4, 8, 12, 16});
         descend("sweep.chunk", [&] { return (uint64_t)t.chunk; },
                 [&](uint64_t v) { t.chunk = (uint32_t)v; }, {2, 8, 32});
         descend("sweep.blocks_per_cu", [&] { return (uint64_t)t.blocks_per_cu; },

@@ -376,8 +376,8 @@ int chk_tune(chk_index* idx, const char* key, int64_t value) {
         if (k == "sweep.blocks_per_cu") t.blocks_per_cu = v;
         else if (k == "sweep.chunk") t.chunk = v;
         else if (k == "sweep.items") {
-            CHK_REQUIRE(v == 0 || v == 4 || v == 8 || v == 16, CHK_ERR_INVALID,
-                        "sweep.items must be 0, 4, 8 or 16");
+            CHK_REQUIRE(v == 0 || v == 4 || v == 8 || v == 12 || v == 16, CHK_ERR_INVALID,
+                        "sweep.items must be 0, 4, 8, 12 or 16");
             t.items = v;
         } else if (k == "sweep.sparse_max") t.sparse_max = (uint64_t)value;
         else if (k == "sweep.method") {

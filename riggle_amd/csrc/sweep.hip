@@ -160,10 +160,10 @@ struct Stage {
     int64_t w[QH];
 };
 
-// 8-item tiles fit 96 VGPRs (5 waves per SIMD) without spills; 16-item tiles
-// need more registers than that
+// 8-item tiles fit 96 VGPRs (5 waves per SIMD) without spills, 12-item tiles
+// 128 (4 waves); 16-item tiles need more registers than that
 template <int ITEMS>
-__global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : 1) void sweep_kernel(SweepArgs a) {
+__global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sweep_kernel(SweepArgs a) {
     constexpr uint32_t IPW = 64u * ITEMS;  // DB elements per wave per tile
     constexpr uint32_t TILE = NT * ITEMS;
     __shared__ unsigned long long hc[FG], hb[FG];
@@ -645,8 +645,9 @@ void query_plan(Query& q) {
     switch (q.items) {
         case 4: occ = sweep_occupancy<4>(); break;
         case 8: occ = sweep_occupancy<8>(); break;
+        case 12: occ = sweep_occupancy<12>(); break;
         case 16: occ = sweep_occupancy<16>(); break;
-        default: CHK_REQUIRE(false, CHK_ERR_INVALID, "sweep.items must be 4, 8 or 16");
+        default: CHK_REQUIRE(false, CHK_ERR_INVALID, "sweep.items must be 4, 8, 12 or 16");
     }
     if (tu.blocks_per_cu) occ = std::min<int>(occ, (int)tu.blocks_per_cu);
     const uint64_t max_grid = (uint64_t)ix.ncu * occ;
@@ -732,6 +733,7 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
         switch (q.items) {
             case 4: launch_sweep<4>(args, q.grid, st); break;
             case 8: launch_sweep<8>(args, q.grid, st); break;
+            case 12: launch_sweep<12>(args, q.grid, st); break;
             default: launch_sweep<16>(args, q.grid, st); break;
         }
         CHK_HIP(hipGetLastError());

@@ -162,7 +162,7 @@ def test_repeat_is_deterministic(chk, orc):
     assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
 
 
-@pytest.mark.parametrize("chunk,items,bpc", [(1, 4, 1), (3, 8, 2), (64, 16, 0), (2, 8, 0)])
+@pytest.mark.parametrize("chunk,items,bpc", [(1, 4, 1), (3, 8, 2), (64, 16, 0), (2, 8, 0), (5, 12, 0)])
 def test_launch_shapes_agree(chk, orc, chunk, items, bpc):
     # small chunks and tiles force work-queue items that span several jobs,
     # query-file switches inside one block and the pipeline's odd tail
