@@ -246,3 +246,46 @@ def test_tune_rejects_unknown_key(chk):
                    np.zeros(1, np.uint32), 1, 1)
     with pytest.raises(chk.ChuckleError):
         ix.tune("sweep.nonsense", 1)
+
+
+def _shift(t, off):
+    return t[0], t[1] + np.uint32(off), t[2] + np.uint32(off), t[3]
+
+
+def test_coordinates_near_int32_max(chk, orc):
+    # event keys 2x + b then need all 32 bits: the widest coordinate field of
+    # the event sort; one interval ends exactly at 2^31 - 1
+    rng = np.random.default_rng(31)
+    F, nq, S, L, span = 6, 3, 3, 5_000, 200_000
+    off = 2**31 - 1 - span - L
+    db = _shift(flatten(random_files(rng, F, S, (500, 2000), span, L)), off)
+    q = _shift(flatten(random_files(rng, nq, S, (200, 1000), span, L)), off)
+    q = tuple(np.append(a, v).astype(np.uint32) for a, v in zip(q, (0, 2**31 - 50, 2**31 - 1, 0)))
+    assert int(q[2].max()) == 2**31 - 1
+    c, bp = gpu_sweep(chk, db, q, S, F, nq)
+    oc, obp = orc.sweep_bruteforce(db, q, F, nq)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+def test_many_segments_sorted_in_five_passes(chk, orc):
+    # 48 dense query files x 30 shards = 1,440 event segments (11 bits) above
+    # 32-bit coordinates: 43 key bits, sorted in passes of 9, 9, 9, 8 and 8
+    rng = np.random.default_rng(32)
+    F, nq, S, L, span = 5, 48, 30, 3_000, 1_000_000
+    off = 2**31 - 1 - span - L
+    db = _shift(flatten(random_files(rng, F, S, (3000, 6000), span, L)), off)
+    q = _shift(flatten(random_files(rng, nq, S, (300, 800), span, L)), off)
+    ix = chk.Index(*db, S, F)
+    ix.tune("sweep.method", 1)
+    c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3], nq))
+    oc, obp = orc.sweep_ranks(db, q, F, nq, S)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+
+
+def test_single_interval_query(chk, orc):
+    # two events: one tile of the event sort, most of it past the end
+    db = (np.array([0, 0], np.uint32), np.array([10, 500], np.uint32),
+          np.array([600, 900], np.uint32), np.array([0, 1], np.uint32))
+    q = tuple(np.array([v], np.uint32) for v in (0, 550, 700, 0))
+    c, bp = gpu_sweep(chk, db, q, 1, 2, 1)
+    assert c.tolist() == [[1, 1]] and bp.tolist() == [[50, 150]]
