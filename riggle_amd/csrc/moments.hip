@@ -92,8 +92,24 @@ __device__ inline double2 cmul(double2 a, double2 b) {
     return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 
+// Twiddles W^e = exp(-2 pi i e / TW), e < TW / 2, from two 64-entry LDS tables
+// (twa[j] = W^j, twb[i] = W^(64 i)): one complex product instead of a load
+// from the TW-entry table in global memory in every pass.
+__device__ inline double2 lds_twiddle(const double2* twa, const double2* twb, uint32_t e) {
+    return cmul(twa[e & 63u], twb[e >> 6]);
+}
+
+// Fills the LDS twiddle tables from the global TW table (the caller's barrier
+// orders them before use).
+__device__ inline void load_lds_twiddles(const double2* tw, double2* twa, double2* twb) {
+    for (uint32_t i = threadIdx.x; i < 128; i += blockDim.x) {
+        if (i < 64) twa[i] = tw[i];
+        else twb[i - 64] = tw[64 * (i - 64)];
+    }
+}
+
 // In-place DIT FFTs of nfft stacked length-M signals in LDS, input in
-// bit-reversed order, output in natural order.  tw[t] = exp(-2 pi i t / TW).
+// bit-reversed order, output in natural order, twiddles from lds_twiddle.
 // Stages are taken two at a time as radix-4 units held in registers (one LDS
 // round trip and one barrier per pair of radix-2 stages); an odd first stage
 // is a plain radix-2 pass.  Unit (k, group G) of stages s, s+1 (h = 2^(s-1)):
@@ -101,7 +117,8 @@ __device__ inline double2 cmul(double2 a, double2 b) {
 // stage s: b0,1 = a0 +- w1 a1, b2,3 = a2 +- w1 a3; stage s+1: c0,2 = b0 +- w2 b2,
 // c1,3 = b1 +- (w2 W_4h^h) b3 with W_4h^h = -i (+i for the inverse).
 template <bool INV>
-__device__ void lds_fft(double2* x, uint32_t M, uint32_t logM, uint32_t nfft, const double2* tw) {
+__device__ void lds_fft(double2* x, uint32_t M, uint32_t logM, uint32_t nfft, const double2* twa,
+                        const double2* twb) {
     uint32_t s = 1;
     if (logM & 1) {
         const uint32_t nb = nfft * (M >> 1);
@@ -123,7 +140,7 @@ __device__ void lds_fft(double2* x, uint32_t M, uint32_t logM, uint32_t nfft, co
             const uint32_t jj = j & ((M >> 2) - 1);
             const uint32_t k = jj & (h - 1);
             const uint32_t base = f * M + ((jj >> (s - 1)) << (s + 1)) + k;
-            double2 w1 = tw[k * st1], w2 = tw[k * st2];
+            double2 w1 = lds_twiddle(twa, twb, k * st1), w2 = lds_twiddle(twa, twb, k * st2);
             if (INV) {
                 w1.y = -w1.y;
                 w2.y = -w2.y;
@@ -225,11 +242,13 @@ __device__ inline void lds_bitrev(double2* xs, uint32_t M, uint32_t logM, uint32
 // Whole autocorrelation of one signal pair in LDS (N <= FFT_MAXM).
 __global__ __launch_bounds__(NTF) void autocorr_small(Rows r) {
     extern __shared__ __align__(16) double2 xs[];
+    __shared__ double2 twa[64], twb[64];
+    load_lds_twiddles(r.tw, twa, twb);
     const uint32_t p = blockIdx.x;
     const PairIn in(r, p);
     for (uint32_t i = threadIdx.x; i < r.N; i += NTF) xs[bitrev(i, r.logN)] = in(i);
     __syncthreads();
-    lds_fft<false>(xs, r.N, r.logN, 1, r.tw);
+    lds_fft<false>(xs, r.N, r.logN, 1, twa, twb);
     for (uint32_t k = threadIdx.x; k <= r.N / 2; k += NTF) {
         const uint32_t kk = (r.N - k) & (r.N - 1);
         const double2 y = pair_power(xs[k], xs[kk]);
@@ -239,7 +258,7 @@ __global__ __launch_bounds__(NTF) void autocorr_small(Rows r) {
     __syncthreads();
     lds_bitrev(xs, r.N, r.logN, 1);
     __syncthreads();
-    lds_fft<true>(xs, r.N, r.logN, 1, r.tw);
+    lds_fft<true>(xs, r.N, r.logN, 1, twa, twb);
     const PairOut out(r, p);
     for (uint32_t i = threadIdx.x; i < r.n; i += NTF) out(i, xs[i]);
 }
@@ -249,6 +268,8 @@ __global__ __launch_bounds__(NTF) void autocorr_small(Rows r) {
 __global__ __launch_bounds__(NTF) void fft_cols_fwd(Rows r, uint32_t N1, uint32_t N2,
                                                     uint32_t logN2, uint32_t CB) {
     extern __shared__ __align__(16) double2 xs[];
+    __shared__ double2 twa[64], twb[64];
+    load_lds_twiddles(r.tw, twa, twb);
     const uint32_t ncb = N1 / CB;
     const uint32_t p = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
     const PairIn in(r, p);
@@ -257,7 +278,7 @@ __global__ __launch_bounds__(NTF) void fft_cols_fwd(Rows r, uint32_t N1, uint32_
         xs[cc * N2 + bitrev(n2, logN2)] = in((uint64_t)(c0 + cc) + (uint64_t)N1 * n2);
     }
     __syncthreads();
-    lds_fft<false>(xs, N2, logN2, CB, r.tw);
+    lds_fft<false>(xs, N2, logN2, CB, twa, twb);
     double2* X = r.X + (size_t)p * r.N;
     for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
         const uint32_t cc = t % CB, k2 = t / CB;
@@ -272,6 +293,8 @@ __global__ __launch_bounds__(NTF) void fft_cols_fwd(Rows r, uint32_t N1, uint32_
 // FFTs, twiddle W_N^(-n1 k2), in place.
 __global__ __launch_bounds__(NTF) void fft_rows(Rows r, uint32_t N1, uint32_t logN1, uint32_t N2) {
     extern __shared__ __align__(16) double2 xs[];
+    __shared__ double2 twa[64], twb[64];
+    load_lds_twiddles(r.tw, twa, twb);
     const uint32_t per = N2 / 2 + 1;
     const uint32_t p = blockIdx.x / per, q = blockIdx.x % per;
     const uint32_t nr = (q == 0 || q == N2 / 2) ? 1 : 2;
@@ -282,7 +305,7 @@ __global__ __launch_bounds__(NTF) void fft_rows(Rows r, uint32_t N1, uint32_t lo
         xs[j * N1 + bitrev(i, logN1)] = X[(size_t)row * N1 + i];
     }
     __syncthreads();
-    lds_fft<false>(xs, N1, logN1, nr, r.tw);
+    lds_fft<false>(xs, N1, logN1, nr, twa, twb);
     if (nr == 2) {
         for (uint32_t k = threadIdx.x; k < N1; k += NTF) {
             const uint32_t kk = N1 + N1 - 1 - k;
@@ -308,7 +331,7 @@ __global__ __launch_bounds__(NTF) void fft_rows(Rows r, uint32_t N1, uint32_t lo
     __syncthreads();
     lds_bitrev(xs, N1, logN1, nr);
     __syncthreads();
-    lds_fft<true>(xs, N1, logN1, nr, r.tw);
+    lds_fft<true>(xs, N1, logN1, nr, twa, twb);
     for (uint32_t t = threadIdx.x; t < nr * N1; t += NTF) {
         const uint32_t j = t >> logN1, n1 = t & (N1 - 1);
         const uint32_t row = j ? N2 - q : q;
@@ -321,6 +344,8 @@ __global__ __launch_bounds__(NTF) void fft_rows(Rows r, uint32_t N1, uint32_t lo
 __global__ __launch_bounds__(NTF) void fft_cols_inv(Rows r, uint32_t N1, uint32_t N2,
                                                     uint32_t logN2, uint32_t CB) {
     extern __shared__ __align__(16) double2 xs[];
+    __shared__ double2 twa[64], twb[64];
+    load_lds_twiddles(r.tw, twa, twb);
     const uint32_t ncb = N1 / CB;
     const uint32_t p = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
     const double2* X = r.X + (size_t)p * r.N;
@@ -329,7 +354,7 @@ __global__ __launch_bounds__(NTF) void fft_cols_inv(Rows r, uint32_t N1, uint32_
         xs[cc * N2 + bitrev(k2, logN2)] = X[(c0 + cc) + (size_t)N1 * k2];
     }
     __syncthreads();
-    lds_fft<true>(xs, N2, logN2, CB, r.tw);
+    lds_fft<true>(xs, N2, logN2, CB, twa, twb);
     const PairOut out(r, p);
     for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
         const uint32_t cc = t % CB, n2 = t / CB;
