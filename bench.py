@@ -256,6 +256,7 @@ def batched_leg(ix, cfg, F, nshards, D):
         ix.sweep_async(qb, dc, db_)
     ix.sync()
     ms_eager = (time.perf_counter() - t) * 1e3 / reps
+    plan = qb.info()  # dense / sparse split under the tuning the graph uses
     graph = ix.graph(qb, dc, db_)
     graph.launch()
     ix.sync()
@@ -273,7 +274,19 @@ def batched_leg(ix, cfg, F, nshards, D):
     for k in ("sweep.sparse_max", "sweep.items", "sweep.chunk", "sweep.blocks_per_cu"):
         ix.tune(k, 0)
     swept = B * D + int(qc.sum())
-    return {"query_files": B, "query_intervals": int(qc.sum()), "intervals_swept": swept,
+    # DB bytes: each dense file streams the DB once (g = 1: files do not share
+    # staged DB tiles); sparse files read only the layered-table windows they
+    # visit, which this figure leaves out
+    nq = int(qc.sum())
+    bytes_batch = 12 * D * plan["dense_files"] + 8 * nq + 16 * B * F
+    bytes_effective = 12 * D * B + 8 * nq + 16 * B * F
+    t_s = ms_graph * 1e-3
+    return {"query_files": B, "query_intervals": nq, "intervals_swept": swept,
+            "dense_files": plan["dense_files"], "dense_intervals": plan["dense_intervals"],
+            "db_stream_group_g": 1,
+            "roofline": {"bytes_batch": bytes_batch, "frac": bytes_batch / (t_s * HBM_PEAK_GBS * 1e9),
+                         "bytes_effective": bytes_effective,
+                         "frac_effective": bytes_effective / (t_s * HBM_PEAK_GBS * 1e9)},
             "batched_graph_ms": ms_graph, "batched_eager_ms": ms_eager,
             "plain_ms_256_launches": ms_plain, "plain_dense_ms_256_launches": ms_plain_dense,
             "batched_swept_per_s": swept / (ms_graph * 1e-3),

@@ -79,6 +79,15 @@ int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t
                      const uint32_t* start, const uint32_t* end, const uint32_t* qfile,
                      chk_query** out);
 int chk_query_free(chk_query* q);
+/*
+ * The batch's sweep plan under the index's current tuning: query files and
+ * intervals kept (on shards of the DB), and how many files (and their
+ * intervals) take the dense merge-path sweep, which streams the whole DB per
+ * file; the others take the sparse pair sweep.  No counterpart in the
+ * reference (its sweep has a single method).
+ */
+int chk_query_info(chk_query* q, uint32_t* nfiles, uint64_t* n, uint32_t* dense_files,
+                   uint64_t* dense_intervals);
 
 /*
  * Overlap sweep: for every (query file r, DB file f) cell,

@@ -53,6 +53,8 @@ def _declare(L):
     L.chk_index_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)]
     L.chk_query_create.argtypes = [vp, u32, u64, vp, vp, vp, vp, C.POINTER(vp)]
     L.chk_query_free.argtypes = [vp]
+    L.chk_query_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(u64), C.POINTER(C.c_uint32),
+                                 C.POINTER(u64)]
     L.chk_sweep.argtypes = [vp, vp, vp, vp]
     L.chk_sweep_async.argtypes = [vp, vp, vp, vp]
     L.chk_dev_alloc.argtypes = [vp, u64, C.POINTER(vp)]
@@ -265,6 +267,15 @@ class Query:
     @property
     def handle(self):
         return self._h
+
+    def info(self):
+        """Sweep plan under the index's current tuning: files and intervals
+        kept, and the files (and intervals) on the dense merge-path sweep."""
+        nf, n, df = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        dn = C.c_uint64()
+        _check(lib().chk_query_info(self._h, C.byref(nf), C.byref(n), C.byref(df), C.byref(dn)))
+        return {"nfiles": nf.value, "n": n.value, "dense_files": df.value,
+                "dense_intervals": dn.value}
 
     def close(self):
         if getattr(self, "_h", None):

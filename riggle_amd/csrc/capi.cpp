@@ -153,6 +153,19 @@ int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t
     });
 }
 
+int chk_query_info(chk_query* q, uint32_t* nfiles, uint64_t* n, uint32_t* dense_files,
+                   uint64_t* dense_intervals) {
+    return guarded([&] {
+        CHK_REQUIRE(q, CHK_ERR_INVALID, "NULL query");
+        set_device(q->q.idx->device);
+        chk::query_plan(q->q);
+        if (nfiles) *nfiles = q->q.nfiles;
+        if (n) *n = q->q.n;
+        if (dense_files) *dense_files = (uint32_t)q->q.dense_rows.size();
+        if (dense_intervals) *dense_intervals = q->q.n_dense;
+    });
+}
+
 int chk_query_free(chk_query* q) {
     return guarded([&] {
         if (!q) return;

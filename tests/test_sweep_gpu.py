@@ -204,7 +204,11 @@ def test_mixed_batch_chooses_per_file(chk, orc):
     q = flatten([random_files(rng, 1, S, n, 4_000_000, 20_000)[0] for n in sizes])
     ix = chk.Index(*db, S, F)
     ix.tune("sweep.sparse_max", 500)
-    c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3], len(sizes)))
+    qq = ix.query(q[0], q[1], q[2], q[3], len(sizes))
+    info = qq.info()  # files above 500 intervals go dense
+    assert info["nfiles"] == len(sizes) and info["n"] == sum(sizes)
+    assert info["dense_files"] == 3 and info["dense_intervals"] == 4000 + 20_000 + 90_000
+    c, bp = ix.sweep(qq)
     oc, obp = orc.sweep_ranks(db, q, F, len(sizes), S)
     assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
 
