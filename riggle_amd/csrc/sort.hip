@@ -252,9 +252,11 @@ __global__ __launch_bounds__(NT) void rs_scatter(const uint64_t* kin, const uint
 // LDS digit counters (ballots give the rank among equal digits of one round),
 // and the waves' counters are prefix-summed once per tile.
 
+constexpr int EV_ITEMS = 16;  // keys per thread per tile (2048-key tiles write shorter digit runs: slower)
+constexpr int EV_TILE = NT * EV_ITEMS;
 constexpr int EV_MAXD = 10;
 constexpr uint32_t EV_BINS = 1u << EV_MAXD;
-constexpr int EV_WKEYS = RS_TILE / (NT / 64);  // keys per wave and tile
+constexpr int EV_WKEYS = EV_TILE / (NT / 64);  // keys per wave and tile
 
 struct EvSrc {
     const uint32_t* segid;
@@ -264,7 +266,7 @@ struct EvSrc {
 };
 
 // The first pass reads each interval once and takes its two events in turn:
-// tile t holds the events of intervals [t * RS_TILE / 2, (t + 1) * RS_TILE / 2)
+// tile t holds the events of intervals [t * EV_TILE / 2, (t + 1) * EV_TILE / 2)
 // (any fixed order of the input is fine for an LSD sort's first pass).
 __device__ inline void interval_keys(const EvSrc& s, size_t i, uint64_t& ks, uint64_t& ke) {
     const uint64_t seg = (uint64_t)s.segid[i] << s.cb;
@@ -280,12 +282,12 @@ __global__ __launch_bounds__(NT) void ev_upsweep(EvSrc src, const uint64_t* keys
     const uint32_t bins = 1u << nbits;
     for (uint32_t d = threadIdx.x; d < bins; d += NT) h[d] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * RS_TILE + threadIdx.x;
-    uint32_t dig[RS_ITEMS];  // all loads issued before the first LDS atomic
+    const size_t base = (size_t)blockIdx.x * EV_TILE + threadIdx.x;
+    uint32_t dig[EV_ITEMS];  // all loads issued before the first LDS atomic
     if (FIRST) {
-        const size_t n = m / 2, i0 = (size_t)blockIdx.x * (RS_TILE / 2) + threadIdx.x;
+        const size_t n = m / 2, i0 = (size_t)blockIdx.x * (EV_TILE / 2) + threadIdx.x;
 #pragma unroll
-        for (int i = 0; i < RS_ITEMS / 2; i++) {
+        for (int i = 0; i < EV_ITEMS / 2; i++) {
             const size_t iv = i0 + (size_t)i * NT;
             uint64_t ks = 0, ke = 0;
             if (iv < n) interval_keys(src, iv, ks, ke);
@@ -294,14 +296,14 @@ __global__ __launch_bounds__(NT) void ev_upsweep(EvSrc src, const uint64_t* keys
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < RS_ITEMS; i++) {
+        for (int i = 0; i < EV_ITEMS; i++) {
             const size_t idx = base + (size_t)i * NT;
             const uint64_t k = idx < m ? keys[idx] : 0ull;
             dig[i] = idx < m ? (uint32_t)(k >> shift) & (bins - 1) : 0xFFFFFFFFu;
         }
     }
 #pragma unroll
-    for (int i = 0; i < RS_ITEMS; i++)
+    for (int i = 0; i < EV_ITEMS; i++)
         if (dig[i] != 0xFFFFFFFFu) atomicAdd(&h[dig[i]], 1u);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < bins; d += NT) hist[(size_t)d * nblocks + blockIdx.x] = h[d];
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin,
     __shared__ uint32_t cnt[NT / 64][EV_BINS];  // per wave digit counts, then tile-local starts
     __shared__ uint32_t lbase[EV_BINS];          // first tile-local slot of each digit
     __shared__ uint32_t gbase[EV_BINS];          // first output position of each digit
-    __shared__ uint64_t stage[RS_TILE];          // the tile in digit order
+    __shared__ uint64_t stage[EV_TILE];          // the tile in digit order
     __shared__ uint32_t lsum[NT / 64 + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t bins = 1u << nbits;
@@ -325,16 +327,16 @@ __global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin,
     }
     __syncthreads();
     const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const size_t tile0 = (size_t)blockIdx.x * RS_TILE;
+    const size_t tile0 = (size_t)blockIdx.x * EV_TILE;
     const size_t w0 = tile0 + (size_t)wv * EV_WKEYS;
-    uint64_t key[RS_ITEMS];
-    uint32_t loc[RS_ITEMS];
-    bool val[RS_ITEMS];
+    uint64_t key[EV_ITEMS];
+    uint32_t loc[EV_ITEMS];
+    bool val[EV_ITEMS];
     if (FIRST) {  // rounds 2q, 2q+1: start and end events of interval iv(q)
         const size_t n = m / 2;
-        const size_t i0 = (size_t)blockIdx.x * (RS_TILE / 2) + (size_t)wv * (EV_WKEYS / 2) + lane;
+        const size_t i0 = (size_t)blockIdx.x * (EV_TILE / 2) + (size_t)wv * (EV_WKEYS / 2) + lane;
 #pragma unroll
-        for (int q = 0; q < RS_ITEMS / 2; q++) {
+        for (int q = 0; q < EV_ITEMS / 2; q++) {
             const size_t iv = i0 + (size_t)q * 64;
             key[2 * q] = key[2 * q + 1] = 0;
             if (iv < n) interval_keys(src, iv, key[2 * q], key[2 * q + 1]);
@@ -342,14 +344,14 @@ __global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin,
         }
     } else {
 #pragma unroll
-        for (int r = 0; r < RS_ITEMS; r++) {  // all loads issued before the ranking rounds
+        for (int r = 0; r < EV_ITEMS; r++) {  // all loads issued before the ranking rounds
             const size_t idx = w0 + (size_t)r * 64 + lane;
             key[r] = idx < m ? kin[idx] : 0ull;
             val[r] = idx < m;
         }
     }
 #pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++) {
+    for (int r = 0; r < EV_ITEMS; r++) {
         const bool valid = val[r];
         const uint64_t k = key[r];
         const uint32_t d = (uint32_t)(k >> shift) & (bins - 1);
@@ -393,7 +395,7 @@ __global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin,
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < RS_ITEMS; r++) {
+    for (int r = 0; r < EV_ITEMS; r++) {
         if (val[r]) {
             const uint32_t d = (uint32_t)(key[r] >> shift) & (bins - 1);
             stage[cnt[wv][d] + loc[r]] = key[r];
@@ -401,7 +403,7 @@ __global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin,
     }
     __syncthreads();
     // write out in tile order: runs of one digit go to consecutive positions
-    const uint32_t nt = (uint32_t)(m - tile0 < (size_t)RS_TILE ? m - tile0 : (size_t)RS_TILE);
+    const uint32_t nt = (uint32_t)(m - tile0 < (size_t)EV_TILE ? m - tile0 : (size_t)EV_TILE);
     const uint64_t cmask = (1ull << src.cb) - 1ull;
     for (uint32_t j = tid; j < nt; j += NT) {
         const uint64_t k = stage[j];
@@ -528,7 +530,7 @@ void sort_events(const uint32_t* segid, const uint32_t* start, const uint32_t* e
     const EvSrc src{segid, start, end, coord_bits};
     const uint32_t bits = coord_bits + seg_bits;
     const uint32_t P = (bits + EV_MAXD - 1) / EV_MAXD;
-    const uint32_t nblocks = (uint32_t)((m + RS_TILE - 1) / RS_TILE);
+    const uint32_t nblocks = (uint32_t)((m + EV_TILE - 1) / EV_TILE);
     if (P > 1) s.keys_alt.reserve(m);
     const size_t hn = (size_t)nblocks << (bits / P + (bits % P ? 1 : 0));
     s.hist.reserve(hn + 1);
