@@ -286,8 +286,10 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
     // window, for one stream kind: no per-row bounds checks, and in a row with
     // no event inside every lane has the same rank, so its depth, count and W
     // terms are scalar.
+    // nvl = valid lanes of the last row (64 for a full span; the last wave of
+    // a tile with events is short by their number)
     auto rows_full = [&](auto end_c, const Stage<ITEMS>& s, uint32_t p0, uint32_t kv, uint32_t rv,
-                         int64_t wvv) {
+                         int64_t wvv, uint32_t nvl) {
         // event key 2y + b is below DB key 2x + 1 iff y + b <= x: compare the
         // window's y + b with the coordinates directly (no key per element);
         // past the events, 0xFFFFFFFF maps to 2^31, above every coordinate
@@ -296,16 +298,17 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
         int64_t wa = 0;
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
+            const bool last = k == ITEMS - 1;
             const uint32_t x = (uint32_t)s.x[k];
             const uint32_t ea = popc(__ballot(kx <= ufl(x)));
-            const uint32_t eb = popc(__ballot(kx <= rdl(x, 63u)));
+            const uint32_t eb = popc(__ballot(kx <= rdl(x, last ? nvl - 1u : 63u)));
             if (ea != ce) {  // rows without events keep the previous row's prefix
                 ce = ea;
                 rsa = rdl(rv, ea);
                 wa = rdl64(wvv, ea);
             }
             if (ea == eb) {
-                add_t(end_c, s.x[k], s.f[k], p0 + ea, rsa, wa);
+                if (!last || lane < nvl) add_t(end_c, s.x[k], s.f[k], p0 + ea, rsa, wa);
             } else {
                 // lanes past event t gain its start count (the key's low bit)
                 // and its signed coordinate; events past a lane past none
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
                     drs += past ? st : 0u;
                     dw += past ? wi : 0;
                 }
-                add_t(end_c, s.x[k], s.f[k], p0 + ea + c, rsa + drs, wa + dw);
+                if (!last || lane < nvl) add_t(end_c, s.x[k], s.f[k], p0 + ea + c, rsa + drs, wa + dw);
             }
         }
     };
@@ -358,18 +361,19 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
             const uint32_t nvw = nd - wbase < IPW ? nd - wbase : IPW;
             const uint32_t kl = (nvw - 1u) >> 6, ll = (nvw - 1u) & 63u;
             uint32_t dkw = 0;
-            if (nvw == IPW) {
-                dkw = rdl(db_key(s.x[ITEMS - 1]), 63u);
+            if (nvw > IPW - 64u) {
+                dkw = rdl(db_key(s.x[ITEMS - 1]), nvw - (IPW - 64u) - 1u);
             } else {
 #pragma unroll
                 for (int k = 0; k < ITEMS; k++)
                     if ((uint32_t)k == kl) dkw = rdl(db_key(s.x[k]), ll);
             }
-            if (popc(__ballot(kv < dkw)) < 64u && nvw == IPW) {
+            if (popc(__ballot(kv < dkw)) < 64u && nvw > IPW - 64u) {
+                const uint32_t nvl = nvw - (IPW - 64u);
                 if (m.is_end)
-                    rows_full(std::true_type{}, s, m.j0 + jlo, kv, rv, wvv);
+                    rows_full(std::true_type{}, s, m.j0 + jlo, kv, rv, wvv, nvl);
                 else
-                    rows_full(std::false_type{}, s, m.j0 + jlo, kv, rv, wvv);
+                    rows_full(std::false_type{}, s, m.j0 + jlo, kv, rv, wvv, nvl);
             } else if (popc(__ballot(kv < dkw)) < 64u) {
                 // Row rank offset c in [ea, eb] and the prefix values at it,
                 // all from the register window: no LDS read in this loop, so
