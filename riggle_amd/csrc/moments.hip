@@ -110,51 +110,95 @@ __device__ inline void load_lds_twiddles(const double2* tw, double2* twa, double
 
 // In-place DIT FFTs of nfft stacked length-M signals in LDS, input in
 // bit-reversed order, output in natural order, twiddles from lds_twiddle.
-// Stages are taken two at a time as radix-4 units held in registers (one LDS
-// round trip and one barrier per pair of radix-2 stages); an odd first stage
-// is a plain radix-2 pass.  Unit (k, group G) of stages s, s+1 (h = 2^(s-1)):
-// (a0..a3) = x[G + k + {0, h, 2h, 3h}], w1 = W_2h^k, w2 = W_4h^k,
-// stage s: b0,1 = a0 +- w1 a1, b2,3 = a2 +- w1 a3; stage s+1: c0,2 = b0 +- w2 b2,
-// c1,3 = b1 +- (w2 W_4h^h) b3 with W_4h^h = -i (+i for the inverse).
+// Stages are taken three at a time as radix-8 units held in registers (one LDS
+// round trip and one barrier per three radix-2 stages), after one radix-2 or
+// radix-4 pass when log2 M is not a multiple of 3.  Unit (k, group G) of
+// stages s..s+2 (h = 2^(s-1)): a_j = x[G + k + j h], j < 8; stage s pairs
+// (j, j+1) with W_2h^k, stage s+1 pairs (j, j+2) with W_4h^(k + (j&1) h),
+// stage s+2 pairs (j, j+4) with W_8h^(k + j h) = W_8h^k W_8^j.
+__device__ inline double2 cadd(double2 a, double2 b) { return {a.x + b.x, a.y + b.y}; }
+__device__ inline double2 csub(double2 a, double2 b) { return {a.x - b.x, a.y - b.y}; }
+
 template <bool INV>
 __device__ void lds_fft(double2* x, uint32_t M, uint32_t logM, uint32_t nfft, const double2* twa,
                         const double2* twb) {
+    auto tw = [&](uint32_t e) {
+        double2 w = lds_twiddle(twa, twb, e);
+        if (INV) w.y = -w.y;
+        return w;
+    };
+    // v * W_4^1 (-i forward, +i inverse)
+    auto rot4 = [](double2 v) { return INV ? double2{-v.y, v.x} : double2{v.y, -v.x}; };
     uint32_t s = 1;
-    if (logM & 1) {
+    const uint32_t rem = logM % 3;
+    if (rem == 1) {
         const uint32_t nb = nfft * (M >> 1);
         for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
             const double2 a = x[2 * j], b = x[2 * j + 1];
-            x[2 * j] = {a.x + b.x, a.y + b.y};
-            x[2 * j + 1] = {a.x - b.x, a.y - b.y};
+            x[2 * j] = cadd(a, b);
+            x[2 * j + 1] = csub(a, b);
         }
         __syncthreads();
         s = 2;
-    }
-    if (logM < 2) return;
-    const uint32_t nq = nfft * (M >> 2);
-    for (; s + 1 <= logM; s += 2) {
-        const uint32_t h = 1u << (s - 1);
-        const uint32_t st1 = TW >> s, st2 = TW >> (s + 1);
+    } else if (rem == 2) {  // stages 1 and 2 (h = 1, all twiddles 1 and W_4^1)
+        const uint32_t nq = nfft * (M >> 2);
         for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) {
-            const uint32_t f = j >> (logM - 2);
-            const uint32_t jj = j & ((M >> 2) - 1);
+            const uint32_t base = 4 * j;
+            const double2 a0 = x[base], a1 = x[base + 1], a2 = x[base + 2], a3 = x[base + 3];
+            const double2 b0 = cadd(a0, a1), b1 = csub(a0, a1), b2 = cadd(a2, a3), b3 = csub(a2, a3);
+            const double2 u3 = rot4(b3);
+            x[base] = cadd(b0, b2);
+            x[base + 2] = csub(b0, b2);
+            x[base + 1] = cadd(b1, u3);
+            x[base + 3] = csub(b1, u3);
+        }
+        __syncthreads();
+        s = 3;
+    }
+    if (logM < 3) return;
+    constexpr double R2 = 0.70710678118654752440;  // 1 / sqrt(2)
+    const uint32_t no = nfft * (M >> 3);
+    for (; s + 2 <= logM; s += 3) {
+        const uint32_t h = 1u << (s - 1);
+        for (uint32_t j = threadIdx.x; j < no; j += blockDim.x) {
+            const uint32_t f = j >> (logM - 3);
+            const uint32_t jj = j & ((M >> 3) - 1);
             const uint32_t k = jj & (h - 1);
-            const uint32_t base = f * M + ((jj >> (s - 1)) << (s + 1)) + k;
-            double2 w1 = lds_twiddle(twa, twb, k * st1), w2 = lds_twiddle(twa, twb, k * st2);
-            if (INV) {
-                w1.y = -w1.y;
-                w2.y = -w2.y;
+            const uint32_t base = f * M + ((jj >> (s - 1)) << (s + 2)) + k;
+            const double2 w1 = tw(k * (TW >> s)), w2 = tw(k * (TW >> (s + 1))),
+                          w3 = tw(k * (TW >> (s + 2)));
+            double2 a[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) a[i] = x[base + i * h];
+#pragma unroll
+            for (int i = 0; i < 8; i += 2) {  // stage s
+                const double2 t = cmul(w1, a[i + 1]);
+                a[i + 1] = csub(a[i], t);
+                a[i] = cadd(a[i], t);
             }
-            const double2 w3 = INV ? double2{-w2.y, w2.x} : double2{w2.y, -w2.x};  // w2 * (-+i)
-            const double2 a0 = x[base], a1 = x[base + h], a2 = x[base + 2 * h], a3 = x[base + 3 * h];
-            const double2 t1 = cmul(w1, a1), t3 = cmul(w1, a3);
-            const double2 b0 = {a0.x + t1.x, a0.y + t1.y}, b1 = {a0.x - t1.x, a0.y - t1.y};
-            const double2 b2 = {a2.x + t3.x, a2.y + t3.y}, b3 = {a2.x - t3.x, a2.y - t3.y};
-            const double2 u2 = cmul(w2, b2), u3 = cmul(w3, b3);
-            x[base] = {b0.x + u2.x, b0.y + u2.y};
-            x[base + 2 * h] = {b0.x - u2.x, b0.y - u2.y};
-            x[base + h] = {b1.x + u3.x, b1.y + u3.y};
-            x[base + 3 * h] = {b1.x - u3.x, b1.y - u3.y};
+            const double2 w2b = rot4(w2);
+#pragma unroll
+            for (int i = 0; i < 8; i += 4) {  // stage s + 1
+                const double2 t0 = cmul(w2, a[i + 2]), t1 = cmul(w2b, a[i + 3]);
+                a[i + 2] = csub(a[i], t0);
+                a[i] = cadd(a[i], t0);
+                a[i + 3] = csub(a[i + 1], t1);
+                a[i + 1] = cadd(a[i + 1], t1);
+            }
+            // stage s + 2: W_8h^k times W_8^i, i < 4
+            const double2 v1 = cmul(w3, INV ? double2{R2, R2} : double2{R2, -R2});
+            const double2 v2 = rot4(w3);
+            const double2 v3 = rot4(v1);
+            const double2 t0 = cmul(w3, a[4]), t1 = cmul(v1, a[5]), t2 = cmul(v2, a[6]),
+                          t3 = cmul(v3, a[7]);
+            x[base] = cadd(a[0], t0);
+            x[base + 4 * h] = csub(a[0], t0);
+            x[base + h] = cadd(a[1], t1);
+            x[base + 5 * h] = csub(a[1], t1);
+            x[base + 2 * h] = cadd(a[2], t2);
+            x[base + 6 * h] = csub(a[2], t2);
+            x[base + 3 * h] = cadd(a[3], t3);
+            x[base + 7 * h] = csub(a[3], t3);
         }
         __syncthreads();
     }
