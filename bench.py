@@ -313,6 +313,7 @@ def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
     log(f"serving: {pool} distinct query files ({int(sizes.sum())} intervals) in "
         f"{time.time() - t:.1f}s")
     srv = riggle_amd.Server(ix, slots=3, max_batch_files=16)
+    srv.reserve(int(sizes.sum()))  # any batch of this pool allocates nothing
     # each distinct file alone (latency by size, after one pass over the pool
     # has grown the slots' buffers), then capacity: closed loop over the pool,
     # twice (the first pass warms up)

@@ -145,7 +145,8 @@ int chk_moments_download(chk_index* idx, uint64_t* total, int64_t* slot_off, uin
  * result.  Per cell: p, llr and flags (bit 0: a record is emitted -- count > 0,
  * the DB file has moments and shares a chromosome with the query; bit 1: the
  * LLR exists).  Cells without bit 0 carry no record (the reference's absent
- * entry, p = 1).  Device arrays; enqueued without final synchronisation.
+ * entry) and are written p = 1, llr = 0.  Device arrays; enqueued without
+ * final synchronisation.
  */
 int chk_pvalues_async(chk_index* idx, chk_query* q, const uint64_t* d_counts, const uint64_t* d_bp,
                       double* d_p, double* d_llr, uint8_t* d_flags);
@@ -249,6 +250,12 @@ int chk_server_wait(chk_server* s, uint64_t ticket, uint64_t* counts, uint64_t* 
 int chk_server_stats(chk_server* s, uint64_t* completed, double* p50_ms, double* p99_ms,
                      double* mean_ms, double* swept_per_s, double* files_per_batch);
 int chk_server_reset_stats(chk_server* s);
+/* Grow every slot's buffers for batches of up to max_batch_intervals
+ * intervals (and max_batch_files files) while the server is idle, so that
+ * later batches up to that size allocate nothing: growing a device buffer
+ * mid-stream frees the old one, which waits for the whole device and stalls
+ * every other slot.  No counterpart in the reference. */
+int chk_server_reserve(chk_server* s, uint64_t max_batch_intervals);
 /* Waits for in-flight batches, then stops the loop. */
 int chk_server_free(chk_server* s);
 

@@ -83,6 +83,7 @@ def _declare(L):
     L.chk_server_wait.argtypes = [vp, u64, vp, vp, vp, vp, vp, dp_]
     L.chk_server_stats.argtypes = [vp, C.POINTER(u64), dp_, dp_, dp_, dp_, dp_]
     L.chk_server_reset_stats.argtypes = [vp]
+    L.chk_server_reserve.argtypes = [vp, u64]
     L.chk_server_free.argtypes = [vp]
 
 
@@ -328,6 +329,11 @@ class Server:
 
     def reset_stats(self):
         _check(lib().chk_server_reset_stats(self._h))
+
+    def reserve(self, max_batch_intervals):
+        """Grow every slot for batches of up to this many intervals (idle
+        server only), so that serving allocates nothing afterwards."""
+        _check(lib().chk_server_reserve(self._h, int(max_batch_intervals)))
 
     def close(self):
         if getattr(self, "_h", None):

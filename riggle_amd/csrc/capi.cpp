@@ -542,6 +542,14 @@ int chk_server_stats(chk_server* s, uint64_t* completed, double* p50_ms, double*
     });
 }
 
+int chk_server_reserve(chk_server* s, uint64_t max_batch_intervals) {
+    return guarded([&] {
+        CHK_REQUIRE(s && s->s, CHK_ERR_INVALID, "NULL server");
+        set_device(s->idx->ix.device);
+        chk::server_reserve(*s->s, max_batch_intervals);
+    });
+}
+
 int chk_server_reset_stats(chk_server* s) {
     return guarded([&] {
         CHK_REQUIRE(s && s->s, CHK_ERR_INVALID, "NULL server");

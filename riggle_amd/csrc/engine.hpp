@@ -272,6 +272,11 @@ void server_wait(Server& s, uint64_t ticket, uint64_t* counts, uint64_t* bp, dou
 void server_stats(Server& s, uint64_t* completed, double* p50, double* p99, double* mean,
                   double* swept_per_s, double* files_per_batch);
 void server_reset_stats(Server& s);
+void server_reserve(Server& s, uint64_t max_batch_intervals);
+// Grows a query's buffers for batches of up to n intervals in `files` files
+// (the plan's for the current tuning), so that such batches allocate nothing.
+void query_reserve(Query& q, uint64_t n, uint32_t files);
+void plan_reserve(Query& q, uint64_t n, uint32_t files);
 // Plain copy of `bytes` (reads and writes that many bytes once).
 void calibrate_copy(Index& ix, uint64_t bytes);
 

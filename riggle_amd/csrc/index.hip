@@ -440,6 +440,29 @@ bool file_is_dense(const Index& ix, uint64_t n, double visits) {
     return SPARSE_WEIGHT * visits > 2.0 * (double)ix.n + 2.0 * (double)n;
 }
 
+void query_reserve(Query& q, uint64_t n, uint32_t files) {
+    q.st_shard.reserve(n);
+    q.st_start.reserve(n);
+    q.st_end.reserve(n);
+    q.c_shard.reserve(n);
+    q.c_start.reserve(n);
+    q.c_end.reserve(n);
+    q.shard.reserve(n);
+    q.start.reserve(n);
+    q.end.reserve(n);
+    q.row.reserve(n);
+    q.segid.reserve(n);
+    q.place.reserve(files);
+    q.vis_off.reserve((size_t)files + 1);
+    q.vis.reserve(files);
+    q.keys.reserve(2 * n);
+    q.key32.reserve(2 * n + 1);
+    q.rs.reserve(2 * n + 1);
+    q.w.reserve(2 * n + 1);
+    sort_events_reserve(n, q.scratch);
+    plan_reserve(q, n, files);
+}
+
 void query_visits(Query& q) {
     Index& ix = *q.idx;
     const uint32_t R = q.nfiles;

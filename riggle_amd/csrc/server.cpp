@@ -423,6 +423,42 @@ void server_stats(Server& s, uint64_t* completed, double* p50, double* p99, doub
     if (files_per_batch) *files_per_batch = s.batches ? (double)s.batched_files / s.batches : 0.0;
 }
 
+void server_reserve(Server& s, uint64_t max_batch_intervals) {
+    std::lock_guard<std::mutex> g(s.mu);
+    for (auto& sl : s.slots)
+        CHK_REQUIRE(!sl->busy, CHK_ERR_INVALID, "reserve needs an idle server");
+    CHK_REQUIRE(s.pending.empty(), CHK_ERR_INVALID, "reserve needs an idle server");
+    const uint32_t B = s.opt.max_batch_files;
+    const size_t cells = (size_t)B * s.ix->nfiles;
+    for (auto& sl : s.slots) {
+        query_reserve(*sl->q, max_batch_intervals, B);
+        sl->dc.reserve(cells);
+        sl->db.reserve(cells);
+        sl->hc.reserve(cells);
+        sl->hb.reserve(cells);
+        if (s.opt.pvalues) {
+            sl->dp.reserve(cells);
+            sl->dl.reserve(cells);
+            sl->df.reserve(cells);
+            sl->hp.reserve(cells);
+            sl->hl.reserve(cells);
+            sl->hf.reserve(cells);
+            // query length groups: one per distinct 100-bp length per (file,
+            // shard), typically far fewer than the intervals
+            Query& q = *sl->q;
+            const size_t nseg = (size_t)B * s.ix->nshards;
+            const size_t ng = std::min<size_t>(max_batch_intervals, nseg * 1024);
+            q.g_off.reserve(nseg + 1);
+            q.g_lint.reserve(ng);
+            q.g_ci.reserve(ng);
+            q.g_shard.reserve(ng);
+            q.g_ss.reserve(ng);
+            q.g_ss2.reserve(ng);
+            q.g_cnt.reserve(ng);
+        }
+    }
+}
+
 void server_reset_stats(Server& s) {
     std::lock_guard<std::mutex> g(s.mu);
     s.lat_ms.clear();

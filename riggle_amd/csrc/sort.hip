@@ -572,6 +572,16 @@ void sort_events(const uint32_t* segid, const uint32_t* start, const uint32_t* e
     CHK_HIP(hipGetLastError());
 }
 
+void sort_events_reserve(size_t n, SortScratch& s) {
+    const size_t m = 2 * n;
+    const size_t nblocks = (m + EV_TILE - 1) / EV_TILE;
+    const size_t hn = nblocks << EV_MAXD;
+    s.keys_alt.reserve(m);
+    s.hist.reserve(hn + 1);
+    s.sums.reserve(std::max((hn + SC_TILE - 1) / SC_TILE, (m + SC_TILE - 1) / SC_TILE) + 1);
+    s.sums64.reserve((m + SC_TILE - 1) / SC_TILE + 1);
+}
+
 void exclusive_scan_u32(uint32_t* data, size_t n, SortScratch& s, hipStream_t st) {
     s.sums.reserve((n + SC_TILE - 1) / SC_TILE + 1);
     exclusive_scan_impl<uint32_t>(data, n, s.sums.p, st);

@@ -30,6 +30,9 @@ void sort_events(const uint32_t* segid, const uint32_t* start, const uint32_t* e
                  uint32_t seg_bits, uint32_t coord_bits, uint32_t* key32, uint32_t* rs,
                  int64_t* w, uint64_t* keys, SortScratch& s, hipStream_t st);
 
+// Grows s to what sort_events needs for up to n intervals.
+void sort_events_reserve(size_t n, SortScratch& s);
+
 // In-place exclusive scans of data[0..n); data[n] receives the total, so the
 // buffer must hold n+1 elements.
 void exclusive_scan_u32(uint32_t* data, size_t n, SortScratch& s, hipStream_t st);

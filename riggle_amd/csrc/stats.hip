@@ -205,8 +205,12 @@ __global__ __launch_bounds__(256) void pvalue_kernel(StatsArgs a) {
     if (cell >= a.nq * a.F) return;
     const uint32_t r = cell / a.F, f = cell % a.F;
     const uint64_t count = a.counts[cell];
-    if (count == 0 || !a.has_entry[f]) {
-        if (lane == 0) a.flags[cell] = 0;
+    if (count == 0 || !a.has_entry[f]) {  // no record: p = 1 (docs/stats.md:214-216), llr 0
+        if (lane == 0) {
+            a.p[cell] = 1.0;
+            a.llr[cell] = 0.0;
+            a.flags[cell] = 0;
+        }
         return;
     }
     const uint32_t g0 = a.g_off[(size_t)r * a.S], g1 = a.g_off[(size_t)(r + 1) * a.S];
@@ -233,6 +237,8 @@ __global__ __launch_bounds__(256) void pvalue_kernel(StatsArgs a) {
     }
     if (lane != 0) return;
     if (!any) {  // no shared chromosome: no record (stats.rs:61-63)
+        a.p[cell] = 1.0;
+        a.llr[cell] = 0.0;
         a.flags[cell] = 0;
         return;
     }

@@ -600,6 +600,21 @@ int sweep_occupancy() {
 
 }  // namespace
 
+void plan_reserve(Query& q, uint64_t n, uint32_t files) {
+    Index& ix = *q.idx;
+    const uint64_t S = ix.nshards, NG = ix.ngroups;
+    const uint64_t jobs = (uint64_t)files * S * NG * 2;
+    // every dense file's jobs cover both streams of the DB once; a job's
+    // events are its (file, shard) segment, merged with every group and stream
+    const uint64_t tiles = ((uint64_t)files * 2 * ix.n + 4 * NG * n) / ((uint64_t)NT * 4) + jobs;
+    q.jobs.reserve(jobs);
+    q.chunk_job.reserve(tiles + 1);
+    q.part.reserve(tiles + jobs + 1);
+    q.queue.reserve(2);
+    const uint64_t spi_min = ix.tune.sparse_items ? ix.tune.sparse_items : 4;
+    q.sitems.reserve(NG * (n / spi_min + files));
+}
+
 void query_plan(Query& q) {
     Index& ix = *q.idx;
     if (q.plan_version == ix.tune_version) return;

@@ -93,3 +93,22 @@ def test_concurrent_submitters(chk, orc, served):
         oc, obp = oracle_row(orc, db, F, files[i])
         assert (r["counts"].astype(np.int64) == oc).all() and (r["bp"].astype(np.int64) == obp).all()
     srv.close()
+
+
+def test_reserved_server_matches_alone(chk, orc, served):
+    # slots grown up front for the largest batch (with p-values and their
+    # length groups) give the same rows as a server that grows on demand
+    ix, db, F, files = served
+    plain = chk.Server(ix, slots=1, max_batch_files=4, pvalues=True)
+    ref = [plain.wait(plain.submit(*f)) for f in files]
+    plain.close()
+    srv = chk.Server(ix, slots=3, max_batch_files=4, pvalues=True)
+    srv.reserve(sum(len(f[1]) for f in files))
+    tickets = [srv.submit(*f) for f in files]
+    for i, t in enumerate(tickets):
+        r = srv.wait(t)
+        for k in ("counts", "bp", "p", "llr", "flags"):
+            assert (r[k] == ref[i][k]).all(), (i, k)
+    oc, obp = oracle_row(orc, db, F, files[5])
+    assert (ref[5]["counts"].astype(np.int64) == oc).all() and (ref[5]["bp"].astype(np.int64) == obp).all()
+    srv.close()

@@ -85,6 +85,7 @@ def test_no_shared_chrom_no_record(chk, orc):
     out, oc, obp, tabs = run_case(chk, orc, db, q, 1, 1, 2, sizes)
     assert out["counts"][0, 0] == 1
     assert not (out["flags"][0, 0] & 1)
+    assert out["p"][0, 0] == 1.0 and out["llr"][0, 0] == 0.0  # absent record: p = 1
 
 
 def test_fixed_moments_depletion(chk):
