@@ -198,15 +198,23 @@ struct StatsArgs {
     uint8_t* flags;
 };
 
-// One wave per cell; lanes stride over the query file's groups.
+// WPC waves per cell (4 as launched; a block holds 4 waves); the cell's lanes
+// stride over the query file's length groups.  With 4 waves per cell every
+// early exit below is uniform over the block, so the LDS reduction's barrier
+// is reached by all of its threads.
+template <int WPC>
 __global__ __launch_bounds__(256) void pvalue_kernel(StatsArgs a) {
-    const uint32_t cell = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t lane = threadIdx.x & 63;
+    constexpr uint32_t CPB = 4 / WPC;  // cells per block
+    __shared__ double red[4][3];
+    __shared__ uint32_t red_any[4];
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t sub = wv % WPC;  // wave within the cell
+    const uint32_t cell = blockIdx.x * CPB + wv / WPC;
     if (cell >= a.nq * a.F) return;
     const uint32_t r = cell / a.F, f = cell % a.F;
     const uint64_t count = a.counts[cell];
     if (count == 0 || !a.has_entry[f]) {  // no record: p = 1 (docs/stats.md:214-216), llr 0
-        if (lane == 0) {
+        if (sub == 0 && lane == 0) {
             a.p[cell] = 1.0;
             a.llr[cell] = 0.0;
             a.flags[cell] = 0;
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(256) void pvalue_kernel(StatsArgs a) {
     const uint32_t g0 = a.g_off[(size_t)r * a.S], g1 = a.g_off[(size_t)(r + 1) * a.S];
     double mu = 0.0, vr = 0.0, mu_raw = 0.0;
     uint32_t any = 0;
-    for (uint32_t g = g0 + lane; g < g1; g += 64) {
+    for (uint32_t g = g0 + sub * 64 + lane; g < g1; g += WPC * 64) {
         const uint32_t c = a.g_shard[g];
         const int64_t so = a.slot_off[(size_t)f * a.S + c];
         if (so < 0) continue;  // the DB file stores no moments for this shard
@@ -235,7 +243,24 @@ __global__ __launch_bounds__(256) void pvalue_kernel(StatsArgs a) {
         mu_raw += __shfl_xor(mu_raw, d, 64);
         any |= __shfl_xor(any, d, 64);
     }
-    if (lane != 0) return;
+    if (WPC > 1) {
+        if (lane == 0) {
+            red[wv][0] = mu;
+            red[wv][1] = vr;
+            red[wv][2] = mu_raw;
+            red_any[wv] = any;
+        }
+        __syncthreads();
+        if (sub == 0 && lane == 0) {
+            for (uint32_t w = wv + 1; w < wv + WPC; w++) {
+                mu += red[w][0];
+                vr += red[w][1];
+                mu_raw += red[w][2];
+                any |= red_any[w];
+            }
+        }
+    }
+    if (sub != 0 || lane != 0) return;
     if (!any) {  // no shared chromosome: no record (stats.rs:61-63)
         a.p[cell] = 1.0;
         a.llr[cell] = 0.0;
@@ -285,7 +310,10 @@ void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, doubl
                 ix.mom.d_slot_len.p, ix.mom.d_has_entry.p, ix.mom.mean.p, ix.mom.var.p, d_p, d_llr,
                 d_flags};
     hipEvent_t t0 = ix.prof.begin("pvalues", st);
-    hipLaunchKernelGGL(pvalue_kernel, dim3((cells + 3) / 4), dim3(256), 0, st, a);
+    // four waves per cell whatever the batch size: a single query file's
+    // cells still fill the device, and a cell's FP64 sums are taken in the
+    // same order in every batch, so its p-value never depends on the batch
+    hipLaunchKernelGGL(pvalue_kernel<4>, dim3(cells), dim3(256), 0, st, a);
     CHK_HIP(hipGetLastError());
     ix.prof.end("pvalues", t0, st);
 }
