@@ -64,6 +64,10 @@ def _declare(L):
     L.orc_rng_seed_u64.argtypes = [C.c_void_p, C.c_uint64]
     L.orc_rng_next_u32.argtypes = [C.c_void_p]
     L.orc_rng_next_u32.restype = C.c_uint32
+    L.orc_rng_next_u64.argtypes = [C.c_void_p]
+    L.orc_rng_next_u64.restype = C.c_uint64
+    L.orc_rng_range_u32_incl.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+    L.orc_rng_range_u32_incl.restype = C.c_uint32
     L.orc_generate_bed.argtypes = [C.c_uint64, sz, C.c_uint32, C.c_uint32, C.c_uint32, u8p, u32p, u32p]
     L.orc_layer_for_size.argtypes = [C.c_uint32] * 3
     L.orc_layer_for_size.restype = C.c_uint32
@@ -144,6 +148,101 @@ def write_bed(path, chrom, start, end, names=CHROMS24):
     with open(path, "w") as f:
         for c, s, e in zip(chrom.tolist(), start.tolist(), end.tolist()):
             f.write(f"{names[c]}\t{s}\t{e}\n")
+
+
+# --------------------------------------------------------------- Monte Carlo
+
+class StdRng:
+    """rand 0.8.5 StdRng::seed_from_u64 (rng.c), with gen_range on i64 restated
+    in exact integers: UniformInt::sample_single_inclusive draws a u64 v,
+    takes v * range as 128 bits and accepts when the low 64 bits are at most
+    zone = (range << clz64(range)) - 1, returning lo + the high 64 bits."""
+
+    def __init__(self, seed):
+        self._buf = (C.c_uint8 * 128)()
+        lib().orc_rng_seed_u64(C.addressof(self._buf), seed)
+
+    def next_u32(self):
+        return lib().orc_rng_next_u32(C.addressof(self._buf))
+
+    def next_u64(self):
+        return lib().orc_rng_next_u64(C.addressof(self._buf))
+
+    def range_u32(self, lo, hi):
+        return lib().orc_rng_range_u32_incl(C.addressof(self._buf), lo, hi)
+
+    def range_i64(self, lo, hi):
+        rng = (hi - lo + 1) % (1 << 64)
+        if rng == 0:
+            v = self.next_u64()
+            return v - (1 << 64) if v >= 1 << 63 else v
+        zone = ((rng << (64 - rng.bit_length())) - 1) % (1 << 64)
+        while True:
+            m = self.next_u64() * rng
+            if m % (1 << 64) <= zone:
+                return lo + (m >> 64)
+
+
+def hg38_sizes():
+    """src/fourier/genome.rs:7-34."""
+    return dict(HG38)
+
+
+def montecarlo_trials(qmap, seed, milestones, batch_size=100, shuffle=False, whitelist=None):
+    """src/bin/montecarlo.rs:79-163 and :232-311: the trial batches the driver
+    queries.  qmap: {chrom: [(start, end), ...]} in file order.  Yields
+    (trials run in the batch, [non-empty trial: [(chrom, start, end), ...]])."""
+    sizes = hg38_sizes()
+    max_shift = max(sizes.values())
+    ms = sorted(set(milestones))
+    rng = StdRng(seed)
+    total, mi = 0, 0
+    while total < ms[-1]:
+        to_run = min(ms[mi] - total, max(1, batch_size))
+        batch = []
+        for _ in range(to_run):
+            out = []
+            shift = 0 if shuffle else rng.range_i64(-max_shift, max_shift)
+            for chrom in sorted(qmap):
+                if chrom not in sizes:
+                    continue
+                size = sizes[chrom]
+                wl = (whitelist or {}).get(chrom)
+
+                def put(a, b):
+                    if wl is None:
+                        out.append((chrom, a, b))
+                        return
+                    i = 0
+                    while i < len(wl) and wl[i][1] <= a:  # partition_point(end <= a)
+                        i += 1
+                    while i < len(wl) and wl[i][0] < b:
+                        out.append((chrom, max(wl[i][0], a), min(wl[i][1], b)))
+                        i += 1
+
+                for s, e in qmap[chrom]:
+                    if shuffle:
+                        ln = e - s
+                        if ln == 0 or ln > size:
+                            continue
+                        ns = rng.range_u32(0, size - ln)
+                        put(ns, ns + ln)
+                    else:
+                        a, b = s + shift, e + shift
+                        if b <= 0 or a >= size:
+                            continue
+                        a, b = max(a, 0), min(b, size)
+                        if a >= b:
+                            continue
+                        put(a, b)
+            if out:
+                batch.append(out)
+        yield to_run, batch
+        total += to_run
+        while mi < len(ms) and total >= ms[mi]:
+            mi += 1
+        if mi >= len(ms):
+            break
 
 
 # --------------------------------------------------------------------- sweep

@@ -121,3 +121,50 @@ def test_info(orc, tmp_path):
     assert out[5] == "Sources (2):"
     assert sorted(out[6:]) == ["  SID    0: source_0.bed", "  SID    1: source_1.bed"] or \
         sorted(x.split(": ")[1] for x in out[6:]) == ["source_0.bed", "source_1.bed"]
+
+
+def _read_trials(d):
+    out = []
+    for name in sorted(os.listdir(d)):
+        rows = []
+        with open(os.path.join(d, name)) as f:
+            for line in f:
+                c, s, e = line.split("\t")
+                rows.append((c, int(s), int(e)))
+        out.append(rows)
+    return out
+
+
+@pytest.mark.parametrize("mode,wl", [("circular", False), ("shuffle", False), ("circular", True)])
+def test_montecarlo_trials_match_oracle(orc, tmp_path, mode, wl):
+    # src/bin/montecarlo.rs:79-163, 232-311: same seed -> the same shifted /
+    # shuffled trial files, batch by batch (chromosomes in name order, one not
+    # in hg38, intervals reaching past both chromosome ends, a whitelist)
+    q = tmp_path / "q.bed"
+    rows = [("chr2", 5, 400), ("chr1", 248_956_000, 248_956_422), ("chr1", 10, 20_000),
+            ("chrUn", 100, 200), ("chr21", 46_709_000, 46_709_983), ("chr10", 1_000_000, 1_300_000),
+            ("chr1", 100_000_000, 100_000_050), ("chrY", 57_000_000, 57_227_415)]
+    q.write_text("".join(f"{c}\t{s}\t{e}\n" for c, s, e in rows))
+    qmap = {}
+    for c, s, e in rows:
+        qmap.setdefault(c, []).append((s, e))
+    args = ["montecarlo", "-d", str(tmp_path / "nodb"), "-q", str(q), "-t", "7,20,12", "-b", "5",
+            "-s", "1234567", "--mode", mode, "--keep-trials", str(tmp_path / "keep"), "--dry-run"]
+    wmap = None
+    if wl:
+        w = tmp_path / "wl.bed"
+        wrows = [("chr1", 0, 1_000_000), ("chr1", 50_000_000, 150_000_000), ("chr2", 10_000, 90_000_000),
+                 ("chr10", 0, 20_000_000), ("chr21", 1_000_000, 46_000_000)]
+        w.write_text("".join(f"{c}\t{s}\t{e}\n" for c, s, e in wrows))
+        args += ["-w", str(w)]
+        wmap = {}
+        for c, s, e in wrows:
+            wmap.setdefault(c, []).append((s, e))
+    run(*args)
+    batches = list(orc.montecarlo_trials(qmap, 1234567, [7, 20, 12], 5, mode == "shuffle", wmap))
+    assert [b[0] for b in batches] == [5, 2, 5, 5, 3]  # milestones 7, 12, 20 in batches of <= 5
+    for k, (_, trials) in enumerate(batches):
+        d = tmp_path / "keep" / f"batch_{k}"
+        got = _read_trials(d) if d.exists() else []
+        assert got == trials, k
+    assert sum(len(t) for _, b in batches for t in b) > 0

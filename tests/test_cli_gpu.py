@@ -98,3 +98,67 @@ def test_query_file_with_unknown_shards_and_empty_file(orc, tmp_path):
     assert "Query complete: 2 query files" in r.stdout  # the empty file takes no row
     names = {x["query_name"] for x in results(out)}
     assert "query_9.bed" not in names
+
+
+def _rust_exp4(v):
+    m, e = f"{v:.4e}".split("e")
+    return f"{m}e{int(e)}"
+
+
+@pytest.mark.parametrize("mode", ["circular", "shuffle"])
+def test_montecarlo_pvalues_match_oracle(orc, tmp_path, mode):
+    # src/bin/montecarlo.rs:187-356: observed counts, trial batches, exceed
+    # counts (count >= observed; every trial for observed = 0) and the
+    # milestone tables, all recomputed with the oracle's sweep
+    src, qdir = scenario(orc, str(tmp_path), 4, 3000, 1, 2000, 30_000_000)
+    db = str(tmp_path / "db")
+    run("add", "-i", src, "-d", db)
+    q = os.path.join(qdir, "query_0.bed")
+    r = run("montecarlo", "-d", db, "-q", q, "-t", "6,15", "-b", "4", "-s", "77", "--mode", mode)
+    import json
+    meta = json.load(open(os.path.join(db, "meta.json")))
+    names = {int(k): v["name"] for k, v in meta["sid_map"].items()}
+    chroms = {c: i for i, c in enumerate(orc.CHROMS24)}
+    dbf = {}
+    for d, nm in names.items():
+        c, s, e = orc.generate_bed(100 + int(nm.split("_")[1].split(".")[0]), 3000, 30_000_000)
+        dbf[d] = (c.astype(np.uint32), s, e)
+    F = max(names) + 1
+    db_arr = tuple(np.concatenate([dbf[d][k] for d in sorted(dbf)]) for k in range(3))
+    db_arr = db_arr + (np.concatenate([np.full(len(dbf[d][1]), d, np.uint32) for d in sorted(dbf)]),)
+
+    def counts(files):
+        sh, st, en, qf = [], [], [], []
+        for i, rows in enumerate(files):
+            for c, s, e in rows:
+                sh.append(chroms.get(c, 99))
+                st.append(s)
+                en.append(e)
+                qf.append(i)
+        qa = tuple(np.array(v, np.uint32) for v in (sh, st, en, qf))
+        return orc.sweep_ranks(db_arr, qa, F, len(files), 100)[0]
+
+    qc, qs, qe = orc.generate_bed(200, 2000, 30_000_000)
+    qmap = {}
+    for c, s, e in zip(qc.tolist(), qs.tolist(), qe.tolist()):
+        qmap.setdefault(orc.CHROMS24[c], []).append((s, e))
+    obs = counts([[(orc.CHROMS24[c], s, e) for c, s, e in zip(qc.tolist(), qs.tolist(), qe.tolist())]])[0]
+    exceed = np.zeros(F, np.int64)
+    total, lines = 0, []
+    ms = [6, 15]
+    for to_run, trials in orc.montecarlo_trials(qmap, 77, ms, 4, mode == "shuffle"):
+        exceed[obs == 0] += to_run
+        if trials:
+            cnt = counts(trials)
+            exceed += ((cnt >= obs[None, :]) & (obs[None, :] > 0)).sum(axis=0)
+        total += to_run
+        for m in ms:
+            if total >= m and all(f"# milestone={m} " not in x for x in lines):
+                lines.append(f"# milestone={m} trials={total}")
+                lines.append("db_name\tobserved\tp_value")
+                for d in sorted(names):
+                    p = 1.0 if obs[d] == 0 else exceed[d] / total
+                    lines.append(f"{names[d]}\t{obs[d]}\t{_rust_exp4(p)}")
+                lines.append("")
+    assert r.stdout.split("\n")[:len(lines)] == lines
+    assert any(obs > 0)

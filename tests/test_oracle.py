@@ -126,3 +126,30 @@ def test_exact_moments_against_window_brute_force(orc, seed):
         w = (P[L:] - P[:-L]).astype(np.float64) / 100.0
         assert m1[i] == pytest.approx(w.mean(), rel=1e-12)
         assert v1[i] == pytest.approx(w.var(), rel=1e-9, abs=1e-9)
+
+
+def test_stdrng_range_i64_power_of_two_closed_form(orc):
+    # for range = 2^k the zone is (2^k << (63 - k)) - 1 = 2^63 - 1: a draw v
+    # is kept when bit 63 - k of v is clear (the low word of v * 2^k is below
+    # 2^63) and gives lo + (v >> (64 - k)); pins the widening multiply and the
+    # rejection rule
+    r1, r2 = orc.StdRng(99), orc.StdRng(99)
+    rejected = 0
+    for k in (1, 7, 31, 40) * 8:
+        lo = -(1 << (k - 1))
+        got = r1.range_i64(lo, lo + (1 << k) - 1)
+        while True:
+            v = r2.next_u64()
+            if not (v >> (63 - k)) & 1:
+                break
+            rejected += 1
+        assert got == lo + (v >> (64 - k))
+    assert rejected > 0
+
+
+def test_stdrng_range_i64_bounds_and_spread(orc):
+    r = orc.StdRng(5)
+    m = 248_956_422
+    v = [r.range_i64(-m, m) for _ in range(2000)]
+    assert min(v) >= -m and max(v) <= m
+    assert min(v) < -m // 2 and max(v) > m // 2 and abs(sum(v) / len(v)) < m / 10
