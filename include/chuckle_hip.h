@@ -185,6 +185,11 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
  *                          pair path; larger files take the dense path
  *                          (0: |DB| / 1024)
  *   "sweep.method"         0 choose per file by size, 1 dense only, 2 sparse only
+ *   "moments.fft_mib"      FFT workspace of one launch of the moment build's
+ *                          four-step FFT, in MiB (0: the default, 128)
+ *   "moments.fft_block"    complex values one FFT block holds in LDS: 8192
+ *                          (128 KiB, where the device allows it) or 4096
+ *                          (64 KiB, two blocks per CU); 0: 8192
  * sparse_max and method apply to query handles created afterwards; the
  * others to every later sweep.
  */

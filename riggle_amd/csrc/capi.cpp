@@ -397,7 +397,12 @@ int chk_tune(chk_index* idx, const char* key, int64_t value) {
             CHK_REQUIRE(v <= 2, CHK_ERR_INVALID, "sweep.method must be 0, 1 or 2");
             t.method = v;
         } else if (k == "sweep.sparse_items") t.sparse_items = v;
-        else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
+        else if (k == "moments.fft_mib") t.fft_mib = v;
+        else if (k == "moments.fft_block") {
+            CHK_REQUIRE(v == 0 || v == 4096 || v == 8192, CHK_ERR_INVALID,
+                        "moments.fft_block must be 0, 4096 or 8192");
+            t.fft_block = v;
+        } else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
         idx->ix.tune_version++;
     });
 }
@@ -413,6 +418,8 @@ int chk_tune_get(chk_index* idx, const char* key, int64_t* value) {
         else if (k == "sweep.sparse_max") *value = (int64_t)t.sparse_max;
         else if (k == "sweep.method") *value = t.method;
         else if (k == "sweep.sparse_items") *value = t.sparse_items;
+        else if (k == "moments.fft_mib") *value = t.fft_mib;
+        else if (k == "moments.fft_block") *value = t.fft_block;
         else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
     });
 }

@@ -70,6 +70,8 @@ struct Tuning {
     uint64_t sparse_max = 0;     // largest query file (intervals) swept by the sparse path
     uint32_t method = 0;         // 0 auto by size, 1 dense only, 2 sparse only
     uint32_t sparse_items = 0;   // query intervals per sparse work item
+    uint32_t fft_mib = 0;        // moment FFT workspace per launch (MiB), 0 = default
+    uint32_t fft_block = 0;      // complex values per moment FFT block: 0 (= 8192), 4096 or 8192
 };
 
 // Compact sliding-window moment tables of the DB files (FP64, device).

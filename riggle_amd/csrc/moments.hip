@@ -18,18 +18,25 @@
 // The autocorrelation is a complex FFT of the zero-padded H of length
 // N = next_pow2(2n - 1), |X|^2, and the inverse FFT; two signals of the same
 // shard share each complex FFT (one as the real, one as the imaginary part).
-// The FFTs are LDS-staged radix-4 passes: one block per signal pair when N
-// fits in one block's LDS (8192 complex values where the device allows
-// 128 KiB per block), otherwise the four-step split N = N1 * N2 into column
-// FFTs (length N2, twiddled), row FFTs with the power spectrum and the
-// inverse row FFTs fused (a row and its mirror row per block), and inverse
-// column FFTs.
+// The FFTs are radix-8 passes (after one radix-2 or radix-4 pass) through a
+// swizzled, bank-conflict-free LDS buffer; forward transforms are
+// decimation in frequency (natural order in, bit-reversed positions out) and
+// inverse ones either decimation in time from those positions or DIF again
+// with the bit reversal folded into the output addresses, so no pass
+// reorders data.  One block per signal pair when N fits in one block's LDS
+// (8192 complex values where the device allows 128 KiB per block), otherwise
+// the four-step split N = N1 * N2 into column FFTs (length N2), row FFTs with
+// the four-step twiddles, the power spectrum and the inverse row FFTs fused
+// (a row and its mirror row per block), and inverse column FFTs.  Signal
+// pairs go through the three passes in groups whose workspace stays small
+// (128 MiB by default).
 // A is FP64; when every value is within 0.25 of an integer and A[0] equals
 // sum H^2 it is rounded to the exact integers.  Prefix sums are double-double
 // (two-sum compensated), so S2 keeps ~32 significant digits.  Tables are
 // written for the reference's stored block sizes (src/fourier/moments.rs:225-247).
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "engine.hpp"
 
@@ -37,7 +44,7 @@ namespace chk {
 namespace {
 
 constexpr int NT = 256;
-constexpr int NTF = 1024;            // threads of the FFT kernels (16 waves beside one LDS tile)
+constexpr int NTF = 1024;            // most threads of an FFT block (one radix-8 unit each)
 constexpr uint32_t LOG_MAXM = 13;
 constexpr uint32_t FFT_MAXM = 1u << LOG_MAXM;  // largest LDS-resident FFT (complex FP64)
 constexpr uint32_t TW = FFT_MAXM;              // twiddle table period
@@ -108,100 +115,244 @@ __device__ inline void load_lds_twiddles(const double2* tw, double2* twa, double
     }
 }
 
-// In-place DIT FFTs of nfft stacked length-M signals in LDS, input in
-// bit-reversed order, output in natural order, twiddles from lds_twiddle.
-// Stages are taken three at a time as radix-8 units held in registers (one LDS
-// round trip and one barrier per three radix-2 stages), after one radix-2 or
-// radix-4 pass when log2 M is not a multiple of 3.  Unit (k, group G) of
-// stages s..s+2 (h = 2^(s-1)): a_j = x[G + k + j h], j < 8; stage s pairs
-// (j, j+1) with W_2h^k, stage s+1 pairs (j, j+2) with W_4h^(k + (j&1) h),
-// stage s+2 pairs (j, j+4) with W_8h^(k + j h) = W_8h^k W_8^j.
 __device__ inline double2 cadd(double2 a, double2 b) { return {a.x + b.x, a.y + b.y}; }
 __device__ inline double2 csub(double2 a, double2 b) { return {a.x - b.x, a.y - b.y}; }
 
-template <bool INV>
-__device__ void lds_fft(double2* x, uint32_t M, uint32_t logM, uint32_t nfft, const double2* twa,
-                        const double2* twb) {
-    auto tw = [&](uint32_t e) {
-        double2 w = lds_twiddle(twa, twb, e);
-        if (INV) w.y = -w.y;
-        return w;
-    };
-    // v * W_4^1 (-i forward, +i inverse)
-    auto rot4 = [](double2 v) { return INV ? double2{-v.y, v.x} : double2{v.y, -v.x}; };
-    uint32_t s = 1;
-    const uint32_t rem = logM % 3;
-    if (rem == 1) {
-        const uint32_t nb = nfft * (M >> 1);
-        for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
-            const double2 a = x[2 * j], b = x[2 * j + 1];
-            x[2 * j] = cadd(a, b);
-            x[2 * j + 1] = csub(a, b);
-        }
-        __syncthreads();
-        s = 2;
-    } else if (rem == 2) {  // stages 1 and 2 (h = 1, all twiddles 1 and W_4^1)
-        const uint32_t nq = nfft * (M >> 2);
-        for (uint32_t j = threadIdx.x; j < nq; j += blockDim.x) {
-            const uint32_t base = 4 * j;
-            const double2 a0 = x[base], a1 = x[base + 1], a2 = x[base + 2], a3 = x[base + 3];
-            const double2 b0 = cadd(a0, a1), b1 = csub(a0, a1), b2 = cadd(a2, a3), b3 = csub(a2, a3);
-            const double2 u3 = rot4(b3);
-            x[base] = cadd(b0, b2);
-            x[base + 2] = csub(b0, b2);
-            x[base + 1] = cadd(b1, u3);
-            x[base + 3] = csub(b1, u3);
-        }
-        __syncthreads();
-        s = 3;
+// LDS layout of nfft stacked length-M FFTs, element e = f * M + n in 16-byte
+// slots: slot(e) = e XOR (a linear function of e's bits 3..6 and of the
+// FFT index f = e >> logM) in bits 0..3.  Each bit only takes in higher bits,
+// so the map is a bijection that keeps every aligned 16-element group in
+// place.  With it every access pattern of the passes below is free of bank
+// conflicts, whether the LDS serves a wave's lanes 8 or 16 at a time: runs of
+// consecutive elements, radix-8 units of span 8 (groups of 8 lanes 64
+// elements apart), units of span 1 (8 consecutive elements per lane), and
+// the first and last passes of column blocks, whose lanes walk the FFT index
+// fastest.
+__device__ inline uint32_t swz_mask(uint32_t e) {  // from bits 3..6 of e
+    return ((e >> 2) & 2u) ^ (((e >> 4) & 1u) * 3u) ^ ((e >> 3) & 4u) ^ ((e >> 3) & 8u);
+}
+__device__ inline uint32_t swz_fmask(uint32_t f) {  // from bits 0..3 of the FFT index
+    return ((f & 1u) << 2) ^ (((f >> 1) & 1u) * 11u) ^ ((f >> 1) & 2u) ^ ((f >> 3) & 1u);
+}
+__device__ inline uint32_t lds_slot(uint32_t e, uint32_t logM) {
+    return e ^ swz_mask(e) ^ (logM >= 4 ? swz_fmask(e >> logM) : 0u);
+}
+
+// Element n of FFT f in the swizzled LDS buffer, as a load and a store.
+struct LdsIO {
+    double2* xs;
+    uint32_t logM;
+    __device__ double2 operator()(uint32_t f, uint32_t n) const { return xs[lds_slot((f << logM) + n, logM)]; }
+    __device__ void operator()(uint32_t f, uint32_t n, double2 v) const {
+        xs[lds_slot((f << logM) + n, logM)] = v;
     }
-    if (logM < 3) return;
+};
+
+// v * W_4^1 (-i forward, +i inverse)
+template <bool INV>
+__device__ inline double2 rot4(double2 v) {
+    return INV ? double2{-v.y, v.x} : double2{v.y, -v.x};
+}
+
+template <bool INV>
+__device__ inline double2 tw_lds(const double2* twa, const double2* twb, uint32_t e) {
+    double2 w = lds_twiddle(twa, twb, e);
+    if (INV) w.y = -w.y;
+    return w;
+}
+
+// Radix-R butterfly of one unit a[i] = x[base + i hp], k = base mod hp,
+// hp = 2^loghp.  DIF (natural order in, bit-reversed out): radix-2 stages of
+// span R/2 hp down to hp, each twiddling the difference.  DIT (bit-reversed in,
+// natural out): spans hp up to R/2 hp, each twiddling the second input.
+// Stage of span s uses W_(2s)^(k + j hp) for the unit's pair j; with
+// w_s = W_(2s)^k that is w_s times W_4^1 or W_8^j.
+template <int R, bool DIF, bool INV>
+__device__ inline void butterfly(double2 (&a)[R], uint32_t k, uint32_t loghp, const double2* twa,
+                                 const double2* twb) {
     constexpr double R2 = 0.70710678118654752440;  // 1 / sqrt(2)
-    const uint32_t no = nfft * (M >> 3);
-    for (; s + 2 <= logM; s += 3) {
-        const uint32_t h = 1u << (s - 1);
-        for (uint32_t j = threadIdx.x; j < no; j += blockDim.x) {
-            const uint32_t f = j >> (logM - 3);
-            const uint32_t jj = j & ((M >> 3) - 1);
-            const uint32_t k = jj & (h - 1);
-            const uint32_t base = f * M + ((jj >> (s - 1)) << (s + 2)) + k;
-            const double2 w1 = tw(k * (TW >> s)), w2 = tw(k * (TW >> (s + 1))),
-                          w3 = tw(k * (TW >> (s + 2)));
-            double2 a[8];
+    const double2 w1 = tw_lds<INV>(twa, twb, k << (LOG_MAXM - 1 - loghp));  // W_(2hp)^k
+    if constexpr (R == 2) {
+        if (DIF) {
+            const double2 d = csub(a[0], a[1]);
+            a[0] = cadd(a[0], a[1]);
+            a[1] = cmul(d, w1);
+        } else {
+            const double2 t = cmul(a[1], w1);
+            a[1] = csub(a[0], t);
+            a[0] = cadd(a[0], t);
+        }
+        return;
+    }
+    const double2 w2 = tw_lds<INV>(twa, twb, k << (LOG_MAXM - 2 - loghp));  // W_(4hp)^k
+    const double2 w2b = rot4<INV>(w2);                                      // W_(4hp)^(k+hp)
+    if constexpr (R == 4) {
+        if (DIF) {
+            double2 d = csub(a[0], a[2]);
+            a[0] = cadd(a[0], a[2]);
+            a[2] = cmul(d, w2);
+            d = csub(a[1], a[3]);
+            a[1] = cadd(a[1], a[3]);
+            a[3] = cmul(d, w2b);
 #pragma unroll
-            for (int i = 0; i < 8; i++) a[i] = x[base + i * h];
+            for (int i = 0; i < 4; i += 2) {
+                d = csub(a[i], a[i + 1]);
+                a[i] = cadd(a[i], a[i + 1]);
+                a[i + 1] = cmul(d, w1);
+            }
+        } else {
 #pragma unroll
-            for (int i = 0; i < 8; i += 2) {  // stage s
-                const double2 t = cmul(w1, a[i + 1]);
+            for (int i = 0; i < 4; i += 2) {
+                const double2 t = cmul(a[i + 1], w1);
                 a[i + 1] = csub(a[i], t);
                 a[i] = cadd(a[i], t);
             }
-            const double2 w2b = rot4(w2);
-#pragma unroll
-            for (int i = 0; i < 8; i += 4) {  // stage s + 1
-                const double2 t0 = cmul(w2, a[i + 2]), t1 = cmul(w2b, a[i + 3]);
-                a[i + 2] = csub(a[i], t0);
-                a[i] = cadd(a[i], t0);
-                a[i + 3] = csub(a[i + 1], t1);
-                a[i + 1] = cadd(a[i + 1], t1);
-            }
-            // stage s + 2: W_8h^k times W_8^i, i < 4
-            const double2 v1 = cmul(w3, INV ? double2{R2, R2} : double2{R2, -R2});
-            const double2 v2 = rot4(w3);
-            const double2 v3 = rot4(v1);
-            const double2 t0 = cmul(w3, a[4]), t1 = cmul(v1, a[5]), t2 = cmul(v2, a[6]),
-                          t3 = cmul(v3, a[7]);
-            x[base] = cadd(a[0], t0);
-            x[base + 4 * h] = csub(a[0], t0);
-            x[base + h] = cadd(a[1], t1);
-            x[base + 5 * h] = csub(a[1], t1);
-            x[base + 2 * h] = cadd(a[2], t2);
-            x[base + 6 * h] = csub(a[2], t2);
-            x[base + 3 * h] = cadd(a[3], t3);
-            x[base + 7 * h] = csub(a[3], t3);
+            double2 t = cmul(a[2], w2);
+            a[2] = csub(a[0], t);
+            a[0] = cadd(a[0], t);
+            t = cmul(a[3], w2b);
+            a[3] = csub(a[1], t);
+            a[1] = cadd(a[1], t);
         }
+        return;
+    }
+    // R == 8: span 4hp uses W_(8hp)^k W_8^j, j < 4
+    const double2 w3 = tw_lds<INV>(twa, twb, k << (LOG_MAXM - 3 - loghp));
+    const double2 v1 = cmul(w3, INV ? double2{R2, R2} : double2{R2, -R2});
+    const double2 v[4] = {w3, v1, rot4<INV>(w3), rot4<INV>(v1)};
+    if (DIF) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const double2 d = csub(a[i], a[i + 4]);
+            a[i] = cadd(a[i], a[i + 4]);
+            a[i + 4] = cmul(d, v[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i += 4) {
+            double2 d = csub(a[i], a[i + 2]);
+            a[i] = cadd(a[i], a[i + 2]);
+            a[i + 2] = cmul(d, w2);
+            d = csub(a[i + 1], a[i + 3]);
+            a[i + 1] = cadd(a[i + 1], a[i + 3]);
+            a[i + 3] = cmul(d, w2b);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            const double2 d = csub(a[i], a[i + 1]);
+            a[i] = cadd(a[i], a[i + 1]);
+            a[i + 1] = cmul(d, w1);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i += 2) {
+            const double2 t = cmul(a[i + 1], w1);
+            a[i + 1] = csub(a[i], t);
+            a[i] = cadd(a[i], t);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i += 4) {
+            double2 t = cmul(a[i + 2], w2);
+            a[i + 2] = csub(a[i], t);
+            a[i] = cadd(a[i], t);
+            t = cmul(a[i + 3], w2b);
+            a[i + 3] = csub(a[i + 1], t);
+            a[i + 1] = cadd(a[i + 1], t);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const double2 t = cmul(a[i + 4], v[i]);
+            a[i + 4] = csub(a[i], t);
+            a[i] = cadd(a[i], t);
+        }
+    }
+}
+
+// One radix-R pass over nfft stacked length-M FFTs (span hp = 2^loghp):
+// unit (f, k, group g) reads element g R hp + k + i hp, i < R, of FFT f via
+// ld(f, n), and writes it back via st(f, n, v).  FMAJ: consecutive threads
+// take consecutive FFTs (column blocks: their elements are adjacent in
+// memory), else consecutive units of one FFT.
+template <int R, bool DIF, bool INV, bool FMAJ, class Ld, class St>
+__device__ inline void fft_pass(uint32_t logM, uint32_t lognfft, uint32_t loghp, const Ld& ld, const St& st,
+                                const double2* twa, const double2* twb) {
+    constexpr uint32_t LR = R == 2 ? 1 : (R == 4 ? 2 : 3);
+    constexpr bool LDS_IN = std::is_same<Ld, LdsIO>::value, LDS_OUT = std::is_same<St, LdsIO>::value;
+    const uint32_t logu = logM - LR;  // units per FFT
+    const uint32_t nu = 1u << (logu + lognfft);
+    // LDS slots of a unit: its first element E0 has no bits where i << loghp
+    // has, and the swizzle is linear, so slot(E0 + (i << loghp)) =
+    // slot(E0) ^ c_i with c_i = (i << loghp) ^ swz_mask(i << loghp), the same
+    // for every unit of the pass
+    uint32_t c[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) c[i] = ((uint32_t)i << loghp) ^ swz_mask((uint32_t)i << loghp);
+    for (uint32_t j = threadIdx.x; j < nu; j += blockDim.x) {
+        const uint32_t f = FMAJ ? (j & ((1u << lognfft) - 1u)) : (j >> logu);
+        const uint32_t jj = FMAJ ? (j >> lognfft) : (j & ((1u << logu) - 1u));
+        const uint32_t k = jj & ((1u << loghp) - 1u);
+        const uint32_t base = ((jj >> loghp) << (loghp + LR)) + k;
+        const uint32_t s0 = lds_slot((f << logM) + base, logM);
+        double2 a[R];
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            if constexpr (LDS_IN) a[i] = ld.xs[s0 ^ c[i]];
+            else a[i] = ld(f, base + ((uint32_t)i << loghp));
+        }
+        butterfly<R, DIF, INV>(a, k, loghp, twa, twb);
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            if constexpr (LDS_OUT) st.xs[s0 ^ c[i]] = a[i];
+            else st(f, base + ((uint32_t)i << loghp), a[i]);
+        }
+    }
+}
+
+// DIF FFTs of nfft = 2^lognfft stacked length-2^logM signals: natural order
+// in through in(f, n), out(f, p, v) at bit-reversed positions p, the LDS in
+// between.  The radix-2 or radix-4 remainder of log2 M (largest spans) comes
+// first, then radix-8 passes down to span 1; the first pass reads its unit
+// straight from `in` and the last writes straight to `out`.  FMAJ_IO gives
+// those two passes the FFT-major thread order.
+template <bool INV, bool FMAJ_IO, class In, class Out>
+__device__ void fft_dif(double2* xs, uint32_t logM, uint32_t lognfft, const In& in, const Out& out,
+                        const double2* twa, const double2* twb) {
+    const LdsIO L{xs, logM};
+    const uint32_t rem = logM % 3, lr0 = rem ? rem : 3u, lh0 = logM - lr0;
+    if (lh0 == 0) {  // one pass
+        if (lr0 == 1) fft_pass<2, true, INV, FMAJ_IO>(logM, lognfft, 0, in, out, twa, twb);
+        else if (lr0 == 2) fft_pass<4, true, INV, FMAJ_IO>(logM, lognfft, 0, in, out, twa, twb);
+        else fft_pass<8, true, INV, FMAJ_IO>(logM, lognfft, 0, in, out, twa, twb);
+        return;
+    }
+    if (lr0 == 1) fft_pass<2, true, INV, FMAJ_IO>(logM, lognfft, lh0, in, L, twa, twb);
+    else if (lr0 == 2) fft_pass<4, true, INV, FMAJ_IO>(logM, lognfft, lh0, in, L, twa, twb);
+    else fft_pass<8, true, INV, FMAJ_IO>(logM, lognfft, lh0, in, L, twa, twb);
+    __syncthreads();
+    for (uint32_t lh = lh0 - 3; lh > 0; lh -= 3) {
+        fft_pass<8, true, INV, false>(logM, lognfft, lh, L, L, twa, twb);
         __syncthreads();
     }
+    fft_pass<8, true, INV, FMAJ_IO>(logM, lognfft, 0, L, out, twa, twb);
+}
+
+// DIT FFTs of the stacked signals held in the LDS at bit-reversed positions,
+// written in natural order through out(f, n, v): radix-8 passes from span 1
+// up, then the radix-2 or radix-4 remainder (largest spans).
+template <bool INV, class Out>
+__device__ void fft_dit(double2* xs, uint32_t logM, uint32_t lognfft, const Out& out, const double2* twa,
+                        const double2* twb) {
+    const LdsIO L{xs, logM};
+    const uint32_t rem = logM % 3, n8 = logM / 3;
+    for (uint32_t t = 0; t < n8; t++) {
+        if (t + 1 == n8 && rem == 0) {
+            fft_pass<8, false, INV, false>(logM, lognfft, 3 * t, L, out, twa, twb);
+            return;
+        }
+        fft_pass<8, false, INV, false>(logM, lognfft, 3 * t, L, L, twa, twb);
+        __syncthreads();
+    }
+    if (rem == 1) fft_pass<2, false, INV, false>(logM, lognfft, logM - 1, L, out, twa, twb);
+    else fft_pass<4, false, INV, false>(logM, lognfft, logM - 2, L, out, twa, twb);
 }
 
 struct Rows {
@@ -270,141 +421,144 @@ struct PairOut {
     }
 };
 
-// Natural order -> bit-reversed order of nfft stacked length-M rows in LDS.
-__device__ inline void lds_bitrev(double2* xs, uint32_t M, uint32_t logM, uint32_t nfft) {
-    for (uint32_t t = threadIdx.x; t < nfft * M; t += blockDim.x) {
-        const uint32_t i = t & (M - 1), j = bitrev(i, logM);
-        if (i < j) {
-            const uint32_t o = t - i;
-            const double2 v = xs[o + i];
-            xs[o + i] = xs[o + j];
-            xs[o + j] = v;
-        }
-    }
-}
-
-// Whole autocorrelation of one signal pair in LDS (N <= FFT_MAXM).
+// Whole autocorrelation of one signal pair in LDS (N <= FFT_MAXM): DIF
+// forward FFT, Y = pair_power at the bit-reversed positions (Z[k] at bitrev(k)
+// pairs with Z[N - k]), DIT inverse FFT back to natural order.
 __global__ __launch_bounds__(NTF) void autocorr_small(Rows r) {
     extern __shared__ __align__(16) double2 xs[];
     __shared__ double2 twa[64], twb[64];
     load_lds_twiddles(r.tw, twa, twb);
+    __syncthreads();
     const uint32_t p = blockIdx.x;
     const PairIn in(r, p);
-    for (uint32_t i = threadIdx.x; i < r.N; i += NTF) xs[bitrev(i, r.logN)] = in(i);
+    const LdsIO L{xs, r.logN};
+    fft_dif<false, false>(xs, r.logN, 0, [&](uint32_t, uint32_t n) { return in(n); }, L, twa, twb);
     __syncthreads();
-    lds_fft<false>(xs, r.N, r.logN, 1, twa, twb);
-    for (uint32_t k = threadIdx.x; k <= r.N / 2; k += NTF) {
-        const uint32_t kk = (r.N - k) & (r.N - 1);
-        const double2 y = pair_power(xs[k], xs[kk]);
-        xs[k] = y;
-        xs[kk] = y;
+    for (uint32_t k = threadIdx.x; k <= r.N / 2; k += blockDim.x) {
+        const uint32_t pa = bitrev(k, r.logN), pb = bitrev((r.N - k) & (r.N - 1), r.logN);
+        const double2 y = pair_power(L(0, pa), L(0, pb));
+        L(0, pa, y);
+        L(0, pb, y);
     }
     __syncthreads();
-    lds_bitrev(xs, r.N, r.logN, 1);
-    __syncthreads();
-    lds_fft<true>(xs, r.N, r.logN, 1, twa, twb);
     const PairOut out(r, p);
-    for (uint32_t i = threadIdx.x; i < r.n; i += NTF) out(i, xs[i]);
+    fft_dit<true>(
+        xs, r.logN, 0,
+        [&](uint32_t, uint32_t n, double2 v) {
+            if (n < r.n) out(n, v);
+        },
+        twa, twb);
 }
 
-// Four-step pass 1: CB adjacent columns n1 of one signal pair, length-N2 FFTs
-// over n2 (element n1 + N1*n2), twiddle W_N^(n1 k2), stored at n1 + N1*k2.
-__global__ __launch_bounds__(NTF) void fft_cols_fwd(Rows r, uint32_t N1, uint32_t N2,
-                                                    uint32_t logN2, uint32_t CB) {
+// Column blocks read and write CB adjacent columns (a few dozen bytes per
+// row), so neighbouring blocks share cache lines.  The hardware deals blocks
+// round-robin over the 8 XCDs, each with its own L2: block b runs on XCD b % 8.
+// Logical block (b % 8) * per-XCD count + b / 8 (the remainder spread over the
+// first XCDs) gives each XCD a run of consecutive column groups, whose lines
+// its L2 then fetches once.  The mapping is a bijection for any grid size.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t x = b & 7u, l = b >> 3, q = nb >> 3, r = nb & 7u;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
+// Four-step pass 1: CB = 2^logCB adjacent columns n1 of one signal pair,
+// length-N2 DIF FFTs over n2 (element n1 + N1*n2); output position p holds
+// k2 = bitrev(p), stored at n1 + N1*k2 (the twiddle W_N^(n1 k2) is applied
+// by the row pass, where k2 is fixed).
+__global__ __launch_bounds__(NTF) void fft_cols_fwd(Rows r, uint32_t N1, uint32_t logN2, uint32_t logCB) {
     extern __shared__ __align__(16) double2 xs[];
     __shared__ double2 twa[64], twb[64];
     load_lds_twiddles(r.tw, twa, twb);
-    const uint32_t ncb = N1 / CB;
-    const uint32_t p = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
-    const PairIn in(r, p);
-    for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
-        const uint32_t cc = t % CB, n2 = t / CB;
-        xs[cc * N2 + bitrev(n2, logN2)] = in((uint64_t)(c0 + cc) + (uint64_t)N1 * n2);
-    }
     __syncthreads();
-    lds_fft<false>(xs, N2, logN2, CB, twa, twb);
+    const uint32_t ncb = N1 >> logCB, bl = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t p = bl / ncb, c0 = (bl % ncb) << logCB;
+    const PairIn in(r, p);
     double2* X = r.X + (size_t)p * r.N;
-    for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
-        const uint32_t cc = t % CB, k2 = t / CB;
-        const uint32_t n1 = c0 + cc;
-        X[n1 + (size_t)N1 * k2] = cmul(xs[cc * N2 + k2], twiddle(r, (uint64_t)n1 * k2, false));
-    }
+    fft_dif<false, true>(
+        xs, logN2, logCB, [&](uint32_t f, uint32_t n2) { return in((uint64_t)(c0 + f) + (uint64_t)N1 * n2); },
+        [&](uint32_t f, uint32_t pos, double2 v) { X[(c0 + f) + (size_t)N1 * bitrev(pos, logN2)] = v; },
+        twa, twb);
 }
 
 // Four-step pass 2: the rows k2 and N2 - k2 of one signal pair (row k2 holds
 // Z[k2 + N2 k1], so Z[N - k] of its entry k1 is entry N1 - 1 - k1 of row
-// N2 - k2; rows 0 and N2/2 pair with themselves): forward FFTs, Y, inverse
-// FFTs, twiddle W_N^(-n1 k2), in place.
+// N2 - k2, which after the DIF sits at the complementary bit-reversed
+// position; rows 0 and N2/2 pair with themselves): twiddle W_N^(n1 k2) on
+// load, forward FFTs, Y, inverse FFTs, twiddle W_N^(-n1 k2) on store, in
+// place.  The row's twiddles come from two 64-entry LDS factors,
+// W_N^(n1 k2) = W_N^(64 a k2) W_N^(b k2) for n1 = 64 a + b.
 __global__ __launch_bounds__(NTF) void fft_rows(Rows r, uint32_t N1, uint32_t logN1, uint32_t N2) {
     extern __shared__ __align__(16) double2 xs[];
     __shared__ double2 twa[64], twb[64];
+    __shared__ double2 rtw[2][128];  // per row: [a] = W_N^(64 a k2), [64 + b] = W_N^(b k2)
     load_lds_twiddles(r.tw, twa, twb);
     const uint32_t per = N2 / 2 + 1;
     const uint32_t p = blockIdx.x / per, q = blockIdx.x % per;
     const uint32_t nr = (q == 0 || q == N2 / 2) ? 1 : 2;
+    for (uint32_t t = threadIdx.x; t < nr * 128; t += blockDim.x) {
+        const uint32_t j = t >> 7, i = t & 127u;
+        const uint64_t row = j ? N2 - q : q;
+        if (i >= 64 || 64 * i < N1) rtw[j][i] = twiddle(r, (i < 64 ? 64ull * i : i - 64ull) * row, false);
+    }
+    __syncthreads();
+    auto rowtw = [&](uint32_t f, uint32_t n) { return cmul(rtw[f][n >> 6], rtw[f][64 + (n & 63u)]); };
     double2* X = r.X + (size_t)p * r.N;
-    for (uint32_t t = threadIdx.x; t < nr * N1; t += NTF) {
-        const uint32_t j = t >> logN1, i = t & (N1 - 1);
-        const uint32_t row = j ? N2 - q : q;
-        xs[j * N1 + bitrev(i, logN1)] = X[(size_t)row * N1 + i];
-    }
+    const LdsIO L{xs, logN1};
+    fft_dif<false, false>(
+        xs, logN1, nr - 1,
+        [&](uint32_t f, uint32_t n) { return cmul(X[(size_t)(f ? N2 - q : q) * N1 + n], rowtw(f, n)); }, L,
+        twa, twb);
     __syncthreads();
-    lds_fft<false>(xs, N1, logN1, nr, twa, twb);
     if (nr == 2) {
-        for (uint32_t k = threadIdx.x; k < N1; k += NTF) {
-            const uint32_t kk = N1 + N1 - 1 - k;
-            const double2 y = pair_power(xs[k], xs[kk]);
-            xs[k] = y;
-            xs[kk] = y;
+        for (uint32_t t = threadIdx.x; t < N1; t += blockDim.x) {
+            const double2 y = pair_power(L(0, t), L(1, N1 - 1 - t));
+            L(0, t, y);
+            L(1, N1 - 1 - t, y);
         }
-    } else if (q == 0) {
-        for (uint32_t k = threadIdx.x; k <= N1 / 2; k += NTF) {
-            const uint32_t kk = (N1 - k) & (N1 - 1);
-            const double2 y = pair_power(xs[k], xs[kk]);
-            xs[k] = y;
-            xs[kk] = y;
+    } else if (q == 0) {  // k1 pairs with N1 - k1 (mod N1)
+        for (uint32_t k = threadIdx.x; k <= N1 / 2; k += blockDim.x) {
+            const uint32_t pa = bitrev(k, logN1), pb = bitrev((N1 - k) & (N1 - 1), logN1);
+            const double2 y = pair_power(L(0, pa), L(0, pb));
+            L(0, pa, y);
+            L(0, pb, y);
         }
-    } else {
-        for (uint32_t k = threadIdx.x; k < N1 / 2; k += NTF) {
-            const uint32_t kk = N1 - 1 - k;
-            const double2 y = pair_power(xs[k], xs[kk]);
-            xs[k] = y;
-            xs[kk] = y;
+    } else {  // k1 pairs with N1 - 1 - k1
+        for (uint32_t t = threadIdx.x; t < N1 / 2; t += blockDim.x) {
+            const double2 y = pair_power(L(0, t), L(0, N1 - 1 - t));
+            L(0, t, y);
+            L(0, N1 - 1 - t, y);
         }
     }
     __syncthreads();
-    lds_bitrev(xs, N1, logN1, nr);
-    __syncthreads();
-    lds_fft<true>(xs, N1, logN1, nr, twa, twb);
-    for (uint32_t t = threadIdx.x; t < nr * N1; t += NTF) {
-        const uint32_t j = t >> logN1, n1 = t & (N1 - 1);
-        const uint32_t row = j ? N2 - q : q;
-        X[(size_t)row * N1 + n1] = cmul(xs[t], twiddle(r, (uint64_t)n1 * row, true));
-    }
+    fft_dit<true>(
+        xs, logN1, nr - 1,
+        [&](uint32_t f, uint32_t n1, double2 v) {
+            double2 w = rowtw(f, n1);
+            w.y = -w.y;
+            X[(size_t)(f ? N2 - q : q) * N1 + n1] = cmul(v, w);
+        },
+        twa, twb);
 }
 
-// Four-step pass 3: CB columns, inverse length-N2 FFTs over k2, natural n2;
-// A[n1 + N1*n2] = Re / N (first row), Im / N (second row) below n.
-__global__ __launch_bounds__(NTF) void fft_cols_inv(Rows r, uint32_t N1, uint32_t N2,
-                                                    uint32_t logN2, uint32_t CB) {
+// Four-step pass 3: CB columns, inverse length-N2 DIF FFTs over k2; position
+// p holds n2 = bitrev(p): A[n1 + N1*n2] = Re / N (first row), Im / N (second
+// row) below n.
+__global__ __launch_bounds__(NTF) void fft_cols_inv(Rows r, uint32_t N1, uint32_t logN2, uint32_t logCB) {
     extern __shared__ __align__(16) double2 xs[];
     __shared__ double2 twa[64], twb[64];
     load_lds_twiddles(r.tw, twa, twb);
-    const uint32_t ncb = N1 / CB;
-    const uint32_t p = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * CB;
-    const double2* X = r.X + (size_t)p * r.N;
-    for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
-        const uint32_t cc = t % CB, k2 = t / CB;
-        xs[cc * N2 + bitrev(k2, logN2)] = X[(c0 + cc) + (size_t)N1 * k2];
-    }
     __syncthreads();
-    lds_fft<true>(xs, N2, logN2, CB, twa, twb);
+    const uint32_t ncb = N1 >> logCB, bl = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t p = bl / ncb, c0 = (bl % ncb) << logCB;
+    const double2* X = r.X + (size_t)p * r.N;
     const PairOut out(r, p);
-    for (uint32_t t = threadIdx.x; t < CB * N2; t += NTF) {
-        const uint32_t cc = t % CB, n2 = t / CB;
-        const uint64_t idx = (uint64_t)(c0 + cc) + (uint64_t)N1 * n2;
-        if (idx < r.n) out(idx, xs[cc * N2 + n2]);
-    }
+    fft_dif<true, true>(
+        xs, logN2, logCB, [&](uint32_t f, uint32_t k2) { return X[(c0 + f) + (size_t)N1 * k2]; },
+        [&](uint32_t f, uint32_t pos, double2 v) {
+            const uint64_t idx = (uint64_t)(c0 + f) + (uint64_t)N1 * bitrev(pos, logN2);
+            if (idx < r.n) out(idx, v);
+        },
+        twa, twb);
 }
 
 // ------------------------------------------------------------ binning
@@ -746,11 +900,16 @@ void moments_build(Index& ix) {
     slot_of_fid.reserve(F);
     std::vector<int32_t> sof(F, -1);
     int nexact = 0, nrows = 0;
-    // largest LDS-resident FFT: 8192 complex FP64 (128 KiB) where the device
-    // allows that much LDS per block, else 4096
+    // FFT block: maxm complex FP64 values in LDS, one radix-8 unit per thread:
+    // 8192 (128 KiB, one block per CU) where the device allows that much LDS
+    // per block, else (or when the tuning asks for it) 4096 (64 KiB, two
+    // blocks per CU)
     int max_lds = 0;
     CHK_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ix.device));
-    const uint32_t maxm = max_lds >= (int)(FFT_MAXM * sizeof(double2)) ? FFT_MAXM : FFT_MAXM / 2;
+    const uint32_t maxm = ix.tune.fft_block != FFT_MAXM / 2 && max_lds >= (int)(FFT_MAXM * sizeof(double2)) + 4096
+                              ? FFT_MAXM
+                              : FFT_MAXM / 2;
+    const uint32_t nthr = maxm / 8;
     const int lds_need = (int)(maxm * sizeof(double2));
     if (lds_need > 64 * 1024) {
         CHK_HIP(hipFuncSetAttribute((const void*)autocorr_small, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
@@ -768,7 +927,7 @@ void moments_build(Index& ix) {
         const auto& ls = lists[s];
         const uint32_t nl = (uint32_t)ls.size();
         const uint64_t N = 1ull << ilog2(2ull * n - 1);
-        CHK_REQUIRE(N <= (uint64_t)maxm * maxm / 8, CHK_ERR_INVALID,
+        CHK_REQUIRE(N <= (uint64_t)maxm * maxm / 2, CHK_ERR_INVALID,
                     "shard too long for the moment FFT");
         const uint32_t logN = ilog2(N);
         if (N > maxm) {  // four-step twiddles exp(-2 pi i t / N), t < FFT_MAXM
@@ -803,12 +962,19 @@ void moments_build(Index& ix) {
         // give every CU rows to work on
         size_t free_b = 0, total_b = 0;
         CHK_HIP(hipMemGetInfo(&free_b, &total_b));
-        const uint64_t row_bytes = N * sizeof(double2) / 2 + (uint64_t)n * 12 + 64 * (uint64_t)nl;
+        const uint64_t row_bytes = (uint64_t)n * 12 + 64 * (uint64_t)nl +
+                                   (N <= maxm ? 0 : N * sizeof(double2) / 2);
         const uint32_t B = (uint32_t)std::max<uint64_t>(
             1, std::min<uint64_t>({files.size(), (uint64_t)(free_b / 3) / row_bytes, 4096}));
         const uint32_t ldg = n + 1;
+        // four-step FFTs run over groups of signal pairs whose workspace fits
+        // the budget (by default well inside the 256 MiB last-level cache, so
+        // the three passes re-read it from there rather than from HBM)
+        const uint64_t fft_budget = (uint64_t)(ix.tune.fft_mib ? ix.tune.fft_mib : 128) << 20;
+        const uint32_t PG = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>((B + 1) / 2, fft_budget / (N * sizeof(double2))));
         D.reserve((size_t)B * ldg);
-        X.reserve((size_t)(B + 1) / 2 * N);
+        if (N > maxm) X.reserve((size_t)PG * N);
         A.reserve((size_t)B * n);
         cen.reserve(B);
         sumg.reserve(B);
@@ -844,18 +1010,27 @@ void moments_build(Index& ix) {
             const uint32_t np = (nb + 1) / 2;  // signal pairs
             hipEvent_t t0 = ix.prof.begin("moments_fft", st);
             if (N <= maxm) {
-                hipLaunchKernelGGL(autocorr_small, dim3(np), dim3(NTF), N * sizeof(double2), st, r);
+                hipLaunchKernelGGL(autocorr_small, dim3(np), dim3(std::max<uint32_t>(64, (uint32_t)N / 8)),
+                                   N * sizeof(double2), st, r);
             } else {
                 // rows of N1 = maxm / 2 (two per block), columns of N2 = N / N1 (CB per block)
                 const uint32_t N1 = maxm / 2, N2 = (uint32_t)(N / N1);
                 const uint32_t CB = std::min<uint32_t>(N1, maxm / N2);
                 const size_t lds_col = (size_t)CB * N2 * sizeof(double2);
-                hipLaunchKernelGGL(fft_cols_fwd, dim3(np * (N1 / CB)), dim3(NTF), lds_col, st, r, N1, N2,
-                                   ilog2(N2), CB);
-                hipLaunchKernelGGL(fft_rows, dim3(np * (N2 / 2 + 1)), dim3(NTF), 2 * N1 * sizeof(double2),
-                                   st, r, N1, ilog2(N1), N2);
-                hipLaunchKernelGGL(fft_cols_inv, dim3(np * (N1 / CB)), dim3(NTF), lds_col, st, r, N1, N2,
-                                   ilog2(N2), CB);
+                for (uint32_t p0 = 0; p0 < np; p0 += PG) {
+                    const uint32_t pn = std::min(PG, np - p0);
+                    Rows rg = r;  // pairs p0 .. p0 + pn - 1, workspace from X[0]
+                    rg.G = D.p + (size_t)2 * p0 * ldg;
+                    rg.cen = cen.p + 2 * p0;
+                    rg.A = A.p + (size_t)2 * p0 * n;
+                    rg.nb = std::min(2 * pn, nb - 2 * p0);
+                    hipLaunchKernelGGL(fft_cols_fwd, dim3(pn * (N1 / CB)), dim3(nthr), lds_col, st, rg, N1,
+                                       ilog2(N2), ilog2(CB));
+                    hipLaunchKernelGGL(fft_rows, dim3(pn * (N2 / 2 + 1)), dim3(nthr),
+                                       2 * N1 * sizeof(double2), st, rg, N1, ilog2(N1), N2);
+                    hipLaunchKernelGGL(fft_cols_inv, dim3(pn * (N1 / CB)), dim3(nthr), lds_col, st, rg, N1,
+                                       ilog2(N2), ilog2(CB));
+                }
             }
             CHK_HIP(hipGetLastError());
             ix.prof.end("moments_fft", t0, st);

@@ -85,3 +85,42 @@ def test_gpu_moments_drive_pvalues(chk, orc):
                 n += 1
                 assert abs(mlog10(out["p"][r, f]) - mlog10(exp[0])) < 1e-3
     assert n > 3
+
+
+def test_fft_pair_groups_match_single_launch(chk, orc):
+    # four-step shard (n = 30,000 bins, N = 65,536) with 7 files: a 1 MiB FFT
+    # workspace runs the 4 signal pairs one per launch, the last pair with a
+    # missing odd row; tables equal the one-launch build bit for bit
+    rng = np.random.default_rng(21)
+    F, S = 7, 2
+    sizes = np.array([3_000_000, 2_000], np.uint64)
+    db = flatten(random_files(rng, F, S, (50, 3000), 3_000_000, 20_000))
+    ix1, rows1, _ = build(chk, db, F, S, sizes)
+    ix2 = chk.Index(*db, S, F)
+    ix2.set_genome(sizes)
+    ix2.tune("moments.fft_mib", 1)
+    assert ix2.tune_get("moments.fft_mib") == 1
+    rows2, _ = ix2.build_moments()
+    assert rows1 == rows2 > 0
+    a, b = ix1.download_moments(), ix2.download_moments()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    wm, wv = compare(ix2, orc, db, F, S, sizes)
+    assert wm <= REL and wv <= REL, (wm, wv)
+
+
+@pytest.mark.parametrize("block", [4096, 8192])
+def test_fft_block_sizes_match_oracle(chk, orc, block):
+    # shard 0: n = 30,000 bins (four-step either way, N1 = block / 2);
+    # shard 1: n = 3,000 bins, N = 8192 (one block at 8192, four-step at 4096)
+    rng = np.random.default_rng(33)
+    F, S = 4, 2
+    sizes = np.array([3_000_000, 300_000], np.uint64)
+    db = flatten(random_files(rng, F, S, (50, 3000), 3_000_000, 20_000))
+    ix = chk.Index(*db, S, F)
+    ix.set_genome(sizes)
+    ix.tune("moments.fft_block", block)
+    rows, _ = ix.build_moments()
+    assert rows > 0
+    wm, wv = compare(ix, orc, db, F, S, sizes)
+    assert wm <= REL and wv <= REL, (wm, wv)
