@@ -190,8 +190,13 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
  *   "moments.fft_block"    complex values one FFT block holds in LDS: 8192
  *                          (128 KiB, where the device allows it) or 4096
  *                          (64 KiB, two blocks per CU); 0: 8192
- * sparse_max and method apply to query handles created afterwards; the
- * others to every later sweep.
+ *   "moments.round"        0: round the FFT autocorrelations to the exact
+ *                          integers they are when every value lies within
+ *                          0.25 of one (default); 1: never round; 2: round
+ *                          only when every value already is an integer (a
+ *                          row that is not is scanned again unrounded)
+ * sparse_max and method apply to query handles created afterwards, the
+ * moments keys to later moment builds, the others to every later sweep.
  */
 int chk_tune(chk_index* idx, const char* key, int64_t value);
 /* Current value of a tuning key (0 = automatic choice). */

@@ -402,6 +402,9 @@ int chk_tune(chk_index* idx, const char* key, int64_t value) {
             CHK_REQUIRE(v == 0 || v == 4096 || v == 8192, CHK_ERR_INVALID,
                         "moments.fft_block must be 0, 4096 or 8192");
             t.fft_block = v;
+        } else if (k == "moments.round") {
+            CHK_REQUIRE(v <= 2, CHK_ERR_INVALID, "moments.round must be 0, 1 or 2");
+            t.mom_round = v;
         } else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
         idx->ix.tune_version++;
     });
@@ -420,6 +423,7 @@ int chk_tune_get(chk_index* idx, const char* key, int64_t* value) {
         else if (k == "sweep.sparse_items") *value = t.sparse_items;
         else if (k == "moments.fft_mib") *value = t.fft_mib;
         else if (k == "moments.fft_block") *value = t.fft_block;
+        else if (k == "moments.round") *value = t.mom_round;
         else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
     });
 }

@@ -613,13 +613,16 @@ __device__ T block_incl_scan(T v, T* lds, T& total) {
     return v;
 }
 
-// One block per row: G = inclusive prefix of the impulse row (in place) and
-// the centring constant c = round(sum G / n).
+// One block per row: G = inclusive prefix of the impulse row (in place), the
+// centring constant c = round(sum G / n) and sum H^2 = sum (G - c)^2, exact
+// (from sum G^2 in 128-bit integers), as a double-double.
 __global__ __launch_bounds__(NT) void coverage_rows(int32_t* D, uint32_t ldg, uint32_t n,
-                                                    int64_t* cen, int64_t* sumg) {
+                                                    int64_t* cen, int64_t* sumg, dd* sumh2) {
     __shared__ int64_t lds[NT / 64 + 1];
+    __shared__ unsigned __int128 l2[NT];
     int32_t* row = D + (size_t)blockIdx.x * ldg;
     int64_t carry = 0, sg = 0;
+    unsigned __int128 sg2 = 0;
     for (uint32_t base = 0; base < n; base += NT) {
         const uint32_t i = base + threadIdx.x;
         int64_t v = i < n ? row[i] : 0;
@@ -628,14 +631,21 @@ __global__ __launch_bounds__(NT) void coverage_rows(int32_t* D, uint32_t ldg, ui
         if (i < n) {
             row[i] = (int32_t)incl;
             sg += incl;
+            sg2 += (unsigned __int128)(uint64_t)(incl * incl);
         }
         carry += tot;
     }
     int64_t tot;
     (void)block_incl_scan<int64_t>(sg, lds, tot);
+    l2[threadIdx.x] = sg2;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        unsigned __int128 s2 = 0;
+        for (int t = 0; t < NT; t++) s2 += l2[t];
+        const int64_t c = (int64_t)llround((double)tot / (double)n);
         sumg[blockIdx.x] = tot;
-        cen[blockIdx.x] = (int64_t)llround((double)tot / (double)n);
+        cen[blockIdx.x] = c;
+        sumh2[blockIdx.x] = dd_from_i128((__int128)s2 - 2 * (__int128)c * tot + (__int128)n * c * c);
     }
 }
 
@@ -665,6 +675,8 @@ struct EvalArgs {
     dd* sE1;
     dd* sQ;
     uint32_t* exact;         // [B] 1 when A was rounded to integers
+    const dd* sumh2;         // [B] sum H^2, exact
+    uint32_t round_mode;     // 0 round within 0.25, 1 never round, 2 round only exact values
 };
 
 __device__ inline dd shfl_up_dd(dd v, int d) {
@@ -719,40 +731,26 @@ __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
     __shared__ int64_t ldsi[NT / 64 + 1];
     __shared__ int32_t sG[CH + CH / EK];  // chunk of G, one pad word after every run
     __shared__ double sA[CH + CH / EK];
-    __shared__ uint32_t sh_exact;
     const uint32_t b = blockIdx.x, n = a.n;
     const int32_t* G = a.G + (size_t)b * a.ldg;
     const double* A = a.A + (size_t)b * n;
     const int64_t c = a.cen[b];
     const int64_t PHn = a.sumg[b] - c * (int64_t)n;
-    // exactness of the FFT: round when every value is within 0.25 of an integer
-    // and A[0] equals sum H^2
-    if (threadIdx.x == 0) sh_exact = 1;
-    __syncthreads();
-    {
-        double dev = 0.0;
-        for (uint32_t i = threadIdx.x; i < n; i += NT) dev = fmax(dev, fabs(A[i] - rint(A[i])));
-        if (dev > 0.25) atomicAnd(&sh_exact, 0u);
-        dd h2 = {0, 0};
-        for (uint32_t i = threadIdx.x; i < n; i += NT) {
-            const double h = (double)((int64_t)G[i] - c);
-            h2 = dd_add(h2, dd_from_prod(h, h));
-        }
-        // reduce h2 over the block
-        for (int d = 32; d >= 1; d >>= 1) h2 = dd_add(h2, {__shfl_xor(h2.hi, d, 64), __shfl_xor(h2.lo, d, 64)});
-        __shared__ dd part[NT / 64];
-        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = h2;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            dd s = part[0];
-            for (int i = 1; i < NT / 64; i++) s = dd_add(s, part[i]);
-            if (fabs((s.hi - rint(A[0])) + s.lo) > 0.5) sh_exact = 0;
-        }
-        __syncthreads();
-    }
-    const bool exact = sh_exact != 0;
-    Terms carry = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    int64_t pcarry = 0;
+    __shared__ uint32_t sh_dev;
+    // exactness of the FFT: A is rounded to integers when every value is
+    // within 0.25 of one and A[0] rounds to sum H^2.  The scan assumes it
+    // when A[0] does and checks every value's distance on the way; a row that
+    // fails (not seen in practice) is scanned again without rounding.
+    const dd sh2 = a.sumh2[b];
+    const double A0r = A[0];
+    const double tol = a.round_mode == 2 ? 0.0 : 0.25;
+    bool exact = a.round_mode != 1 && fabs((sh2.hi - rint(A0r)) + sh2.lo) <= 0.5;
+    Terms carry;
+    int64_t pcarry;
+  scan:
+    carry = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    pcarry = 0;
+    double dev = exact ? fabs(A0r - rint(A0r)) : 0.0;
     // Terms of position i (0..n) given P = P[i] = sum_{j<i} G[j] and av = A[i]
     // (rounded when exact; 0 outside 1 <= i < n).
     auto term = [&](uint32_t i, int64_t P, double av) {
@@ -780,7 +778,11 @@ __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
             const uint64_t i = base + j;
             sG[j + j / EK] = i < n ? G[i] : 0;
             double av = (i >= 1 && i < n) ? A[i] : 0.0;
-            if (exact) av = rint(av);
+            if (exact) {
+                const double ar = rint(av);
+                dev = fmax(dev, fabs(av - ar));
+                av = ar;
+            }
             sA[j + j / EK] = av;
         }
         __syncthreads();
@@ -825,10 +827,20 @@ __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
         carry = terms_add(carry, tot);
         pcarry += gtot;
     }
+    if (exact) {
+        if (threadIdx.x == 0) sh_dev = 0;
+        __syncthreads();
+        if (dev > tol) sh_dev = 1;
+        __syncthreads();
+        if (sh_dev) {
+            exact = false;
+            goto scan;
+        }
+    }
     // carry now holds the totals over positions 0..n: PP[n+1], Q[n+1] = Q[n]
     __syncthreads();
     if (threadIdx.x == 0) a.exact[b] = exact ? 1u : 0u;
-    const double A0 = exact ? rint(A[0]) : A[0];
+    const double A0 = exact ? rint(A0r) : A0r;
     const size_t rb = (size_t)b * a.nl;
     const int64_t PPend = carry.pp;
     const dd Qn = carry.q;
@@ -893,7 +905,7 @@ void moments_build(Index& ix) {
     DevBuf<int32_t> slot_of_fid, pmapL, pmapM, D;
     DevBuf<uint8_t> runflag;
     DevBuf<int64_t> cen, sumg, out_off, sPP_L, sPP_M;
-    DevBuf<dd> sC1, sC2, sE1, sQ;
+    DevBuf<dd> sC1, sC2, sE1, sQ, sumh2;
     DevBuf<uint32_t> d_ls, d_exact;
     DevBuf<double2> X;
     DevBuf<double> A;
@@ -978,6 +990,7 @@ void moments_build(Index& ix) {
         A.reserve((size_t)B * n);
         cen.reserve(B);
         sumg.reserve(B);
+        sumh2.reserve(B);
         out_off.reserve(B);
         d_exact.reserve(B);
         sPP_L.reserve((size_t)B * nl);
@@ -1004,7 +1017,8 @@ void moments_build(Index& ix) {
                 hipLaunchKernelGGL(impulses, dim3(grid), dim3(NT), 0, st, ix.l_start.p + soff,
                                    ix.l_end.p + soff, ix.l_fid.p + soff, scnt, slot_of_fid.p, n,
                                    ldg, D.p);
-            hipLaunchKernelGGL(coverage_rows, dim3(nb), dim3(NT), 0, st, D.p, ldg, n, cen.p, sumg.p);
+            hipLaunchKernelGGL(coverage_rows, dim3(nb), dim3(NT), 0, st, D.p, ldg, n, cen.p, sumg.p,
+                               sumh2.p);
             CHK_HIP(hipGetLastError());
             Rows r{D.p, ldg, cen.p, X.p, A.p, nb, n, (uint32_t)N, logN, tw.p, twN.p};
             const uint32_t np = (nb + 1) / 2;  // signal pairs
@@ -1035,7 +1049,8 @@ void moments_build(Index& ix) {
             CHK_HIP(hipGetLastError());
             ix.prof.end("moments_fft", t0, st);
             EvalArgs ea{D.p, ldg, n, nl, A.p, cen.p, sumg.p, pmapL.p, pmapM.p, d_ls.p, out_off.p,
-                        runflag.p, M.mean.p, M.var.p, sPP_L.p, sPP_M.p, sC1.p, sC2.p, sE1.p, sQ.p, d_exact.p};
+                        runflag.p, M.mean.p, M.var.p, sPP_L.p, sPP_M.p, sC1.p, sC2.p, sE1.p, sQ.p, d_exact.p,
+                        sumh2.p, ix.tune.mom_round};
             t0 = ix.prof.begin("moments_eval", st);
             hipLaunchKernelGGL(eval_rows, dim3(nb), dim3(NT), 0, st, ea);
             CHK_HIP(hipGetLastError());

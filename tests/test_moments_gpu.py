@@ -124,3 +124,26 @@ def test_fft_block_sizes_match_oracle(chk, orc, block):
     assert rows > 0
     wm, wv = compare(ix, orc, db, F, S, sizes)
     assert wm <= REL and wv <= REL, (wm, wv)
+
+
+def test_rounding_modes_and_rescan(chk, orc):
+    # mode 1 never rounds; mode 2 starts every row rounded and, since FFT
+    # outputs are not exact integers, rescans it unrounded: its tables equal
+    # mode 1's bit for bit; all three match the oracle
+    rng = np.random.default_rng(41)
+    F, S = 3, 1
+    sizes = np.array([3_000_000], np.uint64)
+    db = flatten(random_files(rng, F, S, (400, 3000), 3_000_000, 50_000))
+    out = {}
+    for mode in (0, 1, 2):
+        ix = chk.Index(*db, S, F)
+        ix.set_genome(sizes)
+        ix.tune("moments.round", mode)
+        rows, exact = ix.build_moments()
+        out[mode] = (rows, exact, ix.download_moments())
+        wm, wv = compare(ix, orc, db, F, S, sizes)
+        assert wm <= REL and wv <= REL, (mode, wm, wv)
+    assert out[0][1] == out[0][0] == F
+    assert out[1][1] == 0 and out[2][1] == 0
+    for x, y in zip(out[1][2], out[2][2]):
+        assert np.array_equal(x, y)
