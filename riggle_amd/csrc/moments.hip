@@ -615,23 +615,57 @@ __device__ T block_incl_scan(T v, T* lds, T& total) {
 
 // One block per row: G = inclusive prefix of the impulse row (in place), the
 // centring constant c = round(sum G / n) and sum H^2 = sum (G - c)^2, exact
-// (from sum G^2 in 128-bit integers), as a double-double.
+// (from sum G^2 in 128-bit integers), as a double-double.  The row goes by
+// in chunks of NT * RK bins staged in LDS (the next chunk's loads in flight
+// meanwhile); a thread scans its run of RK consecutive bins serially, so a
+// chunk costs one block scan.
 __global__ __launch_bounds__(NT) void coverage_rows(int32_t* D, uint32_t ldg, uint32_t n,
                                                     int64_t* cen, int64_t* sumg, dd* sumh2) {
+    constexpr uint32_t RK = 16, RCH = NT * RK;
     __shared__ int64_t lds[NT / 64 + 1];
+    __shared__ int32_t sv[RCH + RCH / RK];  // one pad word after every run
     __shared__ unsigned __int128 l2[NT];
     int32_t* row = D + (size_t)blockIdx.x * ldg;
     int64_t carry = 0, sg = 0;
     unsigned __int128 sg2 = 0;
-    for (uint32_t base = 0; base < n; base += NT) {
-        const uint32_t i = base + threadIdx.x;
-        int64_t v = i < n ? row[i] : 0;
+    int32_t pv[RK];
+    auto fetch = [&](uint32_t base) {
+#pragma unroll
+        for (uint32_t m = 0; m < RK; m++) {
+            const uint32_t i = base + threadIdx.x + m * NT;
+            pv[m] = i < n ? row[i] : 0;
+        }
+    };
+    fetch(0);
+    for (uint32_t base = 0; base < n; base += RCH) {
+        __syncthreads();  // the previous chunk's LDS reads are done
+#pragma unroll
+        for (uint32_t m = 0; m < RK; m++) {
+            const uint32_t j = threadIdx.x + m * NT;
+            sv[j + j / RK] = pv[m];
+        }
+        if (base + RCH < n) fetch(base + RCH);
+        __syncthreads();
+        const uint32_t r0 = threadIdx.x * (RK + 1);
+        int64_t rs = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < RK; k++) rs += sv[r0 + k];  // 0 past n
         int64_t tot;
-        int64_t incl = block_incl_scan<int64_t>(v, lds, tot) + carry;
-        if (i < n) {
-            row[i] = (int32_t)incl;
-            sg += incl;
-            sg2 += (unsigned __int128)(uint64_t)(incl * incl);
+        int64_t run = block_incl_scan<int64_t>(rs, lds, tot) - rs + carry;
+#pragma unroll
+        for (uint32_t k = 0; k < RK; k++) {
+            run += sv[r0 + k];
+            if (base + threadIdx.x * RK + k < n) {
+                sv[r0 + k] = (int32_t)run;
+                sg += run;
+                sg2 += (unsigned __int128)(uint64_t)(run * run);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t m = 0; m < RK; m++) {
+            const uint32_t j = threadIdx.x + m * NT;
+            if (base + j < n) row[base + j] = sv[j + j / RK];
         }
         carry += tot;
     }
@@ -772,12 +806,26 @@ __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
     // pass over the thread's run, one block scan of the run totals, and a
     // second serial pass, only over runs that hold a sampled position, that
     // emits the exclusive prefixes at the stored block sizes
+    // the next chunk's G and A are loaded into registers while this one is
+    // scanned (CH / NT = EK values of each per thread)
+    int32_t pg[EK];
+    double pa[EK];
+    auto fetch = [&](uint64_t base) {
+#pragma unroll
+        for (uint32_t m = 0; m < EK; m++) {
+            const uint64_t i = base + threadIdx.x + m * NT;
+            pg[m] = i < n ? G[i] : 0;
+            pa[m] = (i >= 1 && i < n) ? A[i] : 0.0;
+        }
+    };
+    fetch(0);
     for (uint64_t base = 0; base <= n; base += CH) {
         __syncthreads();  // the previous chunk's LDS reads are done
-        for (uint32_t j = threadIdx.x; j < CH; j += NT) {
-            const uint64_t i = base + j;
-            sG[j + j / EK] = i < n ? G[i] : 0;
-            double av = (i >= 1 && i < n) ? A[i] : 0.0;
+#pragma unroll
+        for (uint32_t m = 0; m < EK; m++) {
+            const uint32_t j = threadIdx.x + m * NT;
+            sG[j + j / EK] = pg[m];
+            double av = pa[m];
             if (exact) {
                 const double ar = rint(av);
                 dev = fmax(dev, fabs(av - ar));
@@ -785,6 +833,7 @@ __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
             }
             sA[j + j / EK] = av;
         }
+        if (base + CH <= n) fetch(base + CH);
         __syncthreads();
         const uint32_t r0 = threadIdx.x * (EK + 1);
         const uint64_t i0 = base + (uint64_t)threadIdx.x * EK;
