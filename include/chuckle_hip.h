@@ -126,7 +126,7 @@ int chk_moments_upload(chk_index* idx, const int64_t* slot_off, const uint32_t* 
  * a genome shard on the device (replaces build_chrom_moments and the moment
  * store of `chuckle add --stats`, src/tasks/build.rs:160-194,
  * src/fourier/moments.rs:129-254): exact integer coverage per 100-bp bin,
- * FFT autocorrelation of the centred signal (LDS-staged radix-2, FP64),
+ * FFT autocorrelation of the centred signal (radix-8 LDS passes, FP64),
  * double-double prefix sums, mean/var per stored block size.
  * exact_rows (optional) receives the number of signals whose autocorrelation
  * was rounded to exact integers, rows the number of signals.
@@ -187,9 +187,10 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
  *   "sweep.method"         0 choose per file by size, 1 dense only, 2 sparse only
  *   "moments.fft_mib"      FFT workspace of one launch of the moment build's
  *                          four-step FFT, in MiB (0: the default, 128)
- *   "moments.fft_block"    complex values one FFT block holds in LDS: 8192
- *                          (128 KiB, where the device allows it) or 4096
- *                          (64 KiB, two blocks per CU); 0: 8192
+ *   "moments.fft_block"    complex values one four-step row block holds in
+ *                          LDS (a row and its mirror): 4096 (64 KiB, two
+ *                          blocks per CU) or 8192 (128 KiB, where the device
+ *                          allows it); 0: 4096
  *   "moments.round"        0: round the FFT autocorrelations to the exact
  *                          integers they are when every value lies within
  *                          0.25 of one (default); 1: never round; 2: round

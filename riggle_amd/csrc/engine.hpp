@@ -71,7 +71,7 @@ struct Tuning {
     uint32_t method = 0;         // 0 auto by size, 1 dense only, 2 sparse only
     uint32_t sparse_items = 0;   // query intervals per sparse work item
     uint32_t fft_mib = 0;        // moment FFT workspace per launch (MiB), 0 = default
-    uint32_t fft_block = 0;      // complex values per moment FFT block: 0 (= 8192), 4096 or 8192
+    uint32_t fft_block = 0;      // complex values per four-step row block: 0 (= 4096), 4096 or 8192
     uint32_t mom_round = 0;      // autocorrelation rounding: 0 auto, 1 never, 2 only exact values
 };
 

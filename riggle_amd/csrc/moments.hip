@@ -961,16 +961,16 @@ void moments_build(Index& ix) {
     slot_of_fid.reserve(F);
     std::vector<int32_t> sof(F, -1);
     int nexact = 0, nrows = 0;
-    // FFT block: maxm complex FP64 values in LDS, one radix-8 unit per thread:
-    // 8192 (128 KiB, one block per CU) where the device allows that much LDS
-    // per block, else (or when the tuning asks for it) 4096 (64 KiB, two
-    // blocks per CU)
+    // FFT blocks hold up to maxm complex FP64 values in LDS, one radix-8 unit
+    // per thread: 8192 (128 KiB, one block per CU) where the device allows
+    // that much LDS per block, else 4096.  Row blocks (a row and its mirror,
+    // 2 * N1 values) hold mrow values: 4096 by default, so that two blocks
+    // share a CU and one's loads overlap the other's passes; column blocks use
+    // maxm, which gives them more adjacent columns per block.
     int max_lds = 0;
     CHK_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ix.device));
-    const uint32_t maxm = ix.tune.fft_block != FFT_MAXM / 2 && max_lds >= (int)(FFT_MAXM * sizeof(double2)) + 4096
-                              ? FFT_MAXM
-                              : FFT_MAXM / 2;
-    const uint32_t nthr = maxm / 8;
+    const uint32_t maxm = max_lds >= (int)(FFT_MAXM * sizeof(double2)) + 8192 ? FFT_MAXM : FFT_MAXM / 2;
+    const uint32_t mrow = ix.tune.fft_block == FFT_MAXM ? maxm : FFT_MAXM / 2;
     const int lds_need = (int)(maxm * sizeof(double2));
     if (lds_need > 64 * 1024) {
         CHK_HIP(hipFuncSetAttribute((const void*)autocorr_small, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
@@ -988,7 +988,7 @@ void moments_build(Index& ix) {
         const auto& ls = lists[s];
         const uint32_t nl = (uint32_t)ls.size();
         const uint64_t N = 1ull << ilog2(2ull * n - 1);
-        CHK_REQUIRE(N <= (uint64_t)maxm * maxm / 2, CHK_ERR_INVALID,
+        CHK_REQUIRE(N <= (uint64_t)mrow * maxm / 2, CHK_ERR_INVALID,
                     "shard too long for the moment FFT");
         const uint32_t logN = ilog2(N);
         if (N > maxm) {  // four-step twiddles exp(-2 pi i t / N), t < FFT_MAXM
@@ -1076,10 +1076,11 @@ void moments_build(Index& ix) {
                 hipLaunchKernelGGL(autocorr_small, dim3(np), dim3(std::max<uint32_t>(64, (uint32_t)N / 8)),
                                    N * sizeof(double2), st, r);
             } else {
-                // rows of N1 = maxm / 2 (two per block), columns of N2 = N / N1 (CB per block)
-                const uint32_t N1 = maxm / 2, N2 = (uint32_t)(N / N1);
+                // rows of N1 = mrow / 2 (two per block), columns of N2 = N / N1 (CB per block)
+                const uint32_t N1 = mrow / 2, N2 = (uint32_t)(N / N1);
                 const uint32_t CB = std::min<uint32_t>(N1, maxm / N2);
                 const size_t lds_col = (size_t)CB * N2 * sizeof(double2);
+                const uint32_t tcol = CB * N2 / 8, trow = N1 / 4;
                 for (uint32_t p0 = 0; p0 < np; p0 += PG) {
                     const uint32_t pn = std::min(PG, np - p0);
                     Rows rg = r;  // pairs p0 .. p0 + pn - 1, workspace from X[0]
@@ -1087,11 +1088,11 @@ void moments_build(Index& ix) {
                     rg.cen = cen.p + 2 * p0;
                     rg.A = A.p + (size_t)2 * p0 * n;
                     rg.nb = std::min(2 * pn, nb - 2 * p0);
-                    hipLaunchKernelGGL(fft_cols_fwd, dim3(pn * (N1 / CB)), dim3(nthr), lds_col, st, rg, N1,
+                    hipLaunchKernelGGL(fft_cols_fwd, dim3(pn * (N1 / CB)), dim3(tcol), lds_col, st, rg, N1,
                                        ilog2(N2), ilog2(CB));
-                    hipLaunchKernelGGL(fft_rows, dim3(pn * (N2 / 2 + 1)), dim3(nthr),
+                    hipLaunchKernelGGL(fft_rows, dim3(pn * (N2 / 2 + 1)), dim3(trow),
                                        2 * N1 * sizeof(double2), st, rg, N1, ilog2(N1), N2);
-                    hipLaunchKernelGGL(fft_cols_inv, dim3(pn * (N1 / CB)), dim3(nthr), lds_col, st, rg, N1,
+                    hipLaunchKernelGGL(fft_cols_inv, dim3(pn * (N1 / CB)), dim3(tcol), lds_col, st, rg, N1,
                                        ilog2(N2), ilog2(CB));
                 }
             }
