@@ -80,10 +80,6 @@ __device__ inline dd dd_mul(dd a, dd b) {
     double hi = p.hi + lo;
     return {hi, lo - (hi - p.hi)};
 }
-__device__ inline dd dd_from_i64(int64_t v) {  // exact
-    double hi = (double)v;
-    return {hi, (double)(v - (int64_t)hi)};
-}
 __device__ inline dd dd_from_i128(__int128 v) {
     double hi = (double)v;
     __int128 r = v - (__int128)hi;
