@@ -200,7 +200,7 @@ struct Query {
     DevBuf<Job> jobs;
     DevBuf<uint32_t> chunk_job;
     DevBuf<uint32_t> part;            // merge-path split (DB index) at each tile boundary
-    DevBuf<uint32_t> queue;           // work-queue counter (zeroed by each sweep; 8 bytes)
+    DevBuf<uint32_t> queue;           // per-XCD work-queue counters (zeroed by each sweep)
     // sparse plan
     std::vector<SparseItem> items_host;
     DevBuf<SparseItem> sitems;

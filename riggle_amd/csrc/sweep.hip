@@ -46,6 +46,7 @@ constexpr int NT = 256;
 constexpr uint32_t QS = 256;    // events staged per tile on the fast path (nq + 1 <= QS)
 constexpr int QH = QS / NT;     // staged events per thread
 constexpr uint32_t MAXC = 64;   // tiles per work item at most
+constexpr uint32_t QSTRIDE = 32;  // u32 words between the per-XCD queue counters (128 B)
 constexpr uint32_t SPI = 256;   // query intervals per sparse work item at most (default)
 
 __device__ inline uint32_t db_key(int32_t x) { return 2u * (uint32_t)x + 1u; }
@@ -444,8 +445,23 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
         }
     };
 
+    // eight queues, one per XCD (block b runs on XCD b % 8): queue x hands
+    // out chunks x, x + 8, ...; a block whose queue is spent moves on to the
+    // next XCD's
+    uint32_t xq = blockIdx.x & 7u;
     for (;;) {
-        if (tid == 0) s_chunk = atomicAdd(a.queue, 1u);
+        if (tid == 0) {
+            uint32_t c = ~0u;
+            for (int tries = 0; tries < 8; tries++) {
+                const uint64_t cc = (uint64_t)atomicAdd(a.queue + xq * QSTRIDE, 1u) * 8u + xq;
+                if (cc < a.nchunks) {
+                    c = (uint32_t)cc;
+                    break;
+                }
+                xq = (xq + 1u) & 7u;
+            }
+            s_chunk = c;
+        }
         __syncthreads();
         const uint32_t c = ufl(s_chunk);
         if (c >= a.nchunks) break;
@@ -610,7 +626,7 @@ void plan_reserve(Query& q, uint64_t n, uint32_t files) {
     q.jobs.reserve(jobs);
     q.chunk_job.reserve(tiles + 1);
     q.part.reserve(tiles + jobs + 1);
-    q.queue.reserve(2);
+    q.queue.reserve(8 * QSTRIDE);
     const uint64_t spi_min = ix.tune.sparse_items ? ix.tune.sparse_items : 4;
     q.sitems.reserve(NG * (n / spi_min + files));
 }
@@ -717,7 +733,7 @@ void query_plan(Query& q) {
     q.jobs.reserve(J.size());
     q.chunk_job.reserve(CJ.size());
     q.part.reserve(std::max<uint32_t>(1, q.nbounds));
-    q.queue.reserve(2);
+    q.queue.reserve(8 * QSTRIDE);
     q.sitems.reserve(I.size());
     if (!J.empty()) {
         CHK_HIP(hipMemcpyAsync(q.jobs.p, J.data(), J.size() * sizeof(Job), hipMemcpyHostToDevice, st));
@@ -736,7 +752,7 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
     const uint32_t F = ix.nfiles;
     const size_t cells = (size_t)q.nfiles * F;
     zero3_u64(d_counts, cells, d_bp, cells, reinterpret_cast<uint64_t*>(q.queue.p),
-              q.jobs_host.empty() ? 0 : 1, st);
+              q.jobs_host.empty() ? 0 : 8 * QSTRIDE / 2, st);
     if (!q.jobs_host.empty()) {
         hipEvent_t t0 = ix.prof.begin("qprep", st);
         query_events(q, st);
