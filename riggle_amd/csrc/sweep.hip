@@ -686,12 +686,14 @@ void query_plan(Query& q) {
     }
     if (tu.blocks_per_cu) occ = std::min<int>(occ, (int)tu.blocks_per_cu);
     const uint64_t max_grid = (uint64_t)ix.ncu * occ;
-    // about 9 chunks per block: the queue then evens out the blocks' ends
-    // while a chunk's pipeline start-up stays a small share of it (measured
-    // on 1 M query intervals vs 145 M DB intervals: 4 chunks per block are
-    // 2 % slower, 16 or more slower still)
+    // about 6 chunks per block: the queues then even out the blocks' ends
+    // while a chunk's pipeline start-up stays a small share of it; sizes of
+    // 4 mod 8 tiles (measured on 1 M query intervals vs 145 M DB intervals:
+    // 12 and 20 tiles run 2-4 % faster than 16 and 24, 28 and up slower)
+    uint64_t cz = tiles / (max_grid * 6);
+    if (cz >= 4) cz = (cz & ~7ull) + 4;
     q.chunk = tu.chunk ? std::min<uint32_t>(tu.chunk, MAXC)
-                       : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(MAXC, tiles / (max_grid * 9)));
+                       : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(MAXC, cz));
     q.nchunks = (uint32_t)((tiles + q.chunk - 1) / q.chunk);
     q.grid = (uint32_t)std::min<uint64_t>(max_grid, q.nchunks);
     auto& CJ = q.chunk_job_host;
