@@ -169,10 +169,33 @@ struct SparseItem {
     uint32_t row, begin, end, group;
 };
 
+// A second stream for work that may overlap the caller's, with the events
+// that fork it from and join it back to the caller's stream (created on first
+// use, on the current device).
+struct ForkStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    ForkStream() = default;
+    ForkStream(const ForkStream&) = delete;
+    ForkStream& operator=(const ForkStream&) = delete;
+    ~ForkStream() {
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    void init() {
+        if (s) return;
+        CHK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        CHK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        CHK_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    }
+};
+
 struct Query {
     Index* idx = nullptr;
     uint32_t nfiles = 0;
     hipStream_t stream = nullptr;     // uploads and plan copies (the index's stream by default)
+    ForkStream aux;                   // the sparse sweep beside the dense one
     uint64_t n_in = 0;                // intervals given
     uint32_t max_end = 0;             // largest end coordinate kept (sizes the event keys)
     uint64_t n = 0;                   // intervals kept (on shards of the DB)

@@ -755,6 +755,29 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
     const size_t cells = (size_t)q.nfiles * F;
     zero3_u64(d_counts, cells, d_bp, cells, reinterpret_cast<uint64_t*>(q.queue.p),
               q.jobs_host.empty() ? 0 : 8 * QSTRIDE / 2, st);
+    // The sparse files' sweep and the dense files' sweep write different rows:
+    // when a batch has both, the sparse kernel runs on a second stream beside
+    // the dense path (forked after the zero fill, joined at the end), so its
+    // latency-bound scans overlap the dense sweep's issue-bound tiles.
+    const bool both = !q.jobs_host.empty() && !q.items_host.empty();
+    hipStream_t sst = st;
+    if (both) {
+        q.aux.init();
+        CHK_HIP(hipEventRecord(q.aux.fork, st));
+        CHK_HIP(hipStreamWaitEvent(q.aux.s, q.aux.fork, 0));
+        sst = q.aux.s;
+    }
+    if (!q.items_host.empty()) {
+        SparseArgs sa{q.sitems.p, q.shard.p, q.start.p, q.end.p, ix.d_lseg.p, ix.ngroups, ix.nlayers, F,
+                      ix.l_start.p, ix.l_end.p, ix.l_fid.p, ix.jt.p,
+                      reinterpret_cast<unsigned long long*>(d_counts),
+                      reinterpret_cast<unsigned long long*>(d_bp)};
+        const uint32_t ni = (uint32_t)q.items_host.size();
+        hipEvent_t t0 = ix.prof.begin("sparse", sst);
+        hipLaunchKernelGGL(sparse_kernel, dim3(ni), dim3(NT), 0, sst, sa);
+        CHK_HIP(hipGetLastError());
+        ix.prof.end("sparse", t0, sst);
+    }
     if (!q.jobs_host.empty()) {
         hipEvent_t t0 = ix.prof.begin("qprep", st);
         query_events(q, st);
@@ -780,16 +803,9 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
         CHK_HIP(hipGetLastError());
         ix.prof.end("sweep", t0, st);
     }
-    if (!q.items_host.empty()) {
-        SparseArgs sa{q.sitems.p, q.shard.p, q.start.p, q.end.p, ix.d_lseg.p, ix.ngroups, ix.nlayers, F,
-                      ix.l_start.p, ix.l_end.p, ix.l_fid.p, ix.jt.p,
-                      reinterpret_cast<unsigned long long*>(d_counts),
-                      reinterpret_cast<unsigned long long*>(d_bp)};
-        const uint32_t ni = (uint32_t)q.items_host.size();
-        hipEvent_t t0 = ix.prof.begin("sparse", st);
-        hipLaunchKernelGGL(sparse_kernel, dim3(ni), dim3(NT), 0, st, sa);
-        CHK_HIP(hipGetLastError());
-        ix.prof.end("sparse", t0, st);
+    if (both) {
+        CHK_HIP(hipEventRecord(q.aux.join, sst));
+        CHK_HIP(hipStreamWaitEvent(st, q.aux.join, 0));
     }
 }
 
