@@ -196,6 +196,10 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
  *                          0.25 of one (default); 1: never round; 2: round
  *                          only when every value already is an integer (a
  *                          row that is not is scanned again unrounded)
+ *   "moments.eval_seg"     bins per independent segment of the moment
+ *                          evaluation's prefix scans, a multiple of 4096
+ *                          (0: 65536; raised so that a chromosome has at
+ *                          most 256 segments)
  * sparse_max and method apply to query handles created afterwards, the
  * moments keys to later moment builds, the others to every later sweep.
  */

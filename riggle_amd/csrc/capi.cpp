@@ -405,6 +405,9 @@ int chk_tune(chk_index* idx, const char* key, int64_t value) {
         } else if (k == "moments.round") {
             CHK_REQUIRE(v <= 2, CHK_ERR_INVALID, "moments.round must be 0, 1 or 2");
             t.mom_round = v;
+        } else if (k == "moments.eval_seg") {
+            CHK_REQUIRE(v % 4096 == 0, CHK_ERR_INVALID, "moments.eval_seg must be a multiple of 4096");
+            t.eval_seg = v;
         } else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
         idx->ix.tune_version++;
     });
@@ -424,6 +427,7 @@ int chk_tune_get(chk_index* idx, const char* key, int64_t* value) {
         else if (k == "moments.fft_mib") *value = t.fft_mib;
         else if (k == "moments.fft_block") *value = t.fft_block;
         else if (k == "moments.round") *value = t.mom_round;
+        else if (k == "moments.eval_seg") *value = t.eval_seg;
         else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
     });
 }

@@ -609,20 +609,30 @@ __device__ T block_incl_scan(T v, T* lds, T& total) {
     return v;
 }
 
+constexpr uint32_t EK = 16;       // consecutive positions per thread in the evaluation scan
+constexpr uint32_t CH = NT * EK;  // positions per chunk
+// A row's positions 0..n are scanned in segments of EVAL_SEG positions (a
+// multiple of CH) by default, at most MAXSEG per row.
+constexpr uint32_t EVAL_SEG = 65536;
+constexpr uint32_t MAXSEG = 256;
+
 // One block per row: G = inclusive prefix of the impulse row (in place), the
 // centring constant c = round(sum G / n) and sum H^2 = sum (G - c)^2, exact
 // (from sum G^2 in 128-bit integers), as a double-double.  The row goes by
 // in chunks of NT * RK bins staged in LDS (the next chunk's loads in flight
 // meanwhile); a thread scans its run of RK consecutive bins serially, so a
-// chunk costs one block scan.
+// chunk costs one block scan.  Also writes the sum of G over each segment of
+// seg bins (seg a multiple of the chunk), gseg[row * nseg + k] for segment k.
 __global__ __launch_bounds__(NT) void coverage_rows(int32_t* D, uint32_t ldg, uint32_t n,
-                                                    int64_t* cen, int64_t* sumg, dd* sumh2) {
+                                                    int64_t* cen, int64_t* sumg, dd* sumh2,
+                                                    uint32_t seg, uint32_t nseg, int64_t* gseg) {
     constexpr uint32_t RK = 16, RCH = NT * RK;
+    static_assert(RCH == CH, "segments are whole chunks of both scans");
     __shared__ int64_t lds[NT / 64 + 1];
     __shared__ int32_t sv[RCH + RCH / RK];  // one pad word after every run
     __shared__ unsigned __int128 l2[NT];
     int32_t* row = D + (size_t)blockIdx.x * ldg;
-    int64_t carry = 0, sg = 0;
+    int64_t carry = 0, sg = 0, sgs = 0;
     unsigned __int128 sg2 = 0;
     int32_t pv[RK];
     auto fetch = [&](uint32_t base) {
@@ -653,9 +663,16 @@ __global__ __launch_bounds__(NT) void coverage_rows(int32_t* D, uint32_t ldg, ui
             run += sv[r0 + k];
             if (base + threadIdx.x * RK + k < n) {
                 sv[r0 + k] = (int32_t)run;
-                sg += run;
+                sgs += run;
                 sg2 += (unsigned __int128)(uint64_t)(run * run);
             }
+        }
+        if ((base + RCH) % seg == 0 || base + RCH >= n) {  // the segment ends with this chunk
+            int64_t segtot;
+            (void)block_incl_scan<int64_t>(sgs, lds, segtot);
+            if (threadIdx.x == 0) gseg[(size_t)blockIdx.x * nseg + base / seg] = segtot;
+            sg += sgs;
+            sgs = 0;
         }
         __syncthreads();
 #pragma unroll
@@ -680,9 +697,24 @@ __global__ __launch_bounds__(NT) void coverage_rows(int32_t* D, uint32_t ldg, ui
 }
 
 // ------------------------------------------------------------ evaluation
+//
+// The table entries need, at the stored block sizes L, prefix sums over the
+// positions i = 0..n of a row: PP (sum of P[i] = sum_{j<i} G[j]), E1 (sum of
+// (P - c i)^2), Q (sum of (PHn - (P - c i))^2), C1 (sum of A[i]) and C2 (sum
+// of A[i] i).  The first three depend on G only and the last two on A only,
+// so two scans of half the width run over the row's segments (seg
+// positions each), each segment started from zero sums (and from P at its
+// first position); eval_fix adds the earlier segments' totals.  When A is
+// rounded every term is an integer or an exact product, so the sums do not
+// depend on where the row is cut.
 
-constexpr uint32_t EK = 16;       // consecutive positions per thread in the evaluation scan
-constexpr uint32_t CH = NT * EK;  // positions per chunk
+struct TermsG {  // sums of P, (P - c i)^2, (PHn - (P - c i))^2
+    int64_t pp;
+    dd e1, q;
+};
+struct TermsA {  // sums of A, A i
+    dd c1, c2;
+};
 
 struct EvalArgs {
     const int32_t* G;
@@ -694,7 +726,7 @@ struct EvalArgs {
     const int32_t* pmapM;    // [n+2] position -> L index sampled at i = n-L+1, or -1
     const uint32_t* ls;      // [nl] stored block sizes
     const int64_t* out_off;  // [B] first table entry of the row's slot
-    const uint8_t* runflag;  // [(n + 1) / EK + 1] 1 when run r holds a sampled position
+    const uint8_t* runflag;  // [(n + 1) / EK + 1] bit 0: run r holds a position L, bit 1: n-L+1
     double* mean;
     double* var;
     // per row scratch of samples: [B][nl] each
@@ -707,129 +739,114 @@ struct EvalArgs {
     uint32_t* exact;         // [B] 1 when A was rounded to integers
     const dd* sumh2;         // [B] sum H^2, exact
     uint32_t round_mode;     // 0 round within 0.25, 1 never round, 2 round only exact values
+    uint32_t seg, nseg;      // positions per segment (a multiple of CH), segments per row
+    const int64_t* gseg;     // [B][nseg] sum of G over each segment's positions
+    TermsG* totg;            // [B][nseg] each segment's G-side totals
+    TermsA* tota;            // [B][nseg] each segment's A-side totals
+    uint32_t* segdev;        // [B][nseg] 1 when a rounded segment held a value off its integer by > tol
+    uint32_t* redo;          // [B] 1 when the row's A is scanned again unrounded
 };
+
+// Whether a row's scan starts rounded: A[0] must round to sum H^2.
+__device__ inline bool row_rounds(const EvalArgs& a, uint32_t b) {
+    const dd sh2 = a.sumh2[b];
+    const double A0r = a.A[(size_t)b * a.n];
+    return a.round_mode != 1 && fabs((sh2.hi - rint(A0r)) + sh2.lo) <= 0.5;
+}
 
 __device__ inline dd shfl_up_dd(dd v, int d) {
     return {__shfl_up(v.hi, d, 64), __shfl_up(v.lo, d, 64)};
 }
-
-// Exclusive block scan of a tuple of dd sums and one int64.
-struct Terms {
-    int64_t pp;
-    dd c1, c2, e1, q;
-};
-__device__ inline Terms terms_add(const Terms& a, const Terms& b) {
-    return {a.pp + b.pp, dd_add(a.c1, b.c1), dd_add(a.c2, b.c2), dd_add(a.e1, b.e1),
-            dd_add(a.q, b.q)};
+__device__ inline TermsG tadd(const TermsG& a, const TermsG& b) {
+    return {a.pp + b.pp, dd_add(a.e1, b.e1), dd_add(a.q, b.q)};
+}
+__device__ inline TermsA tadd(const TermsA& a, const TermsA& b) {
+    return {dd_add(a.c1, b.c1), dd_add(a.c2, b.c2)};
+}
+__device__ inline TermsG tshfl(const TermsG& v, int d) {
+    return {__shfl_up(v.pp, d, 64), shfl_up_dd(v.e1, d), shfl_up_dd(v.q, d)};
+}
+__device__ inline TermsA tshfl(const TermsA& v, int d) {
+    return {shfl_up_dd(v.c1, d), shfl_up_dd(v.c2, d)};
 }
 
-__device__ Terms block_excl_scan_terms(Terms v, Terms* lds, Terms& total) {
+// Exclusive block scan of v; total receives the block's sum.
+template <class T>
+__device__ T block_excl_scan_t(T v, T* lds, T& total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    Terms x = v;
+    T x = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        Terms y = {__shfl_up(x.pp, d, 64), shfl_up_dd(x.c1, d), shfl_up_dd(x.c2, d),
-                   shfl_up_dd(x.e1, d), shfl_up_dd(x.q, d)};
-        if (lane >= d) x = terms_add(x, y);
+        T y = tshfl(x, d);
+        if (lane >= d) x = tadd(x, y);
     }
     if (lane == 63) lds[w] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
-        Terms run = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        T run{};
         for (int i = 0; i < NT / 64; i++) {
-            Terms t = lds[i];
+            T t = lds[i];
             lds[i] = run;
-            run = terms_add(run, t);
+            run = tadd(run, t);
         }
         lds[NT / 64] = run;
     }
     __syncthreads();
-    // exclusive = inclusive(x) - v  ->  compute as wave-exclusive + wave base
-    Terms wex = {__shfl_up(x.pp, 1, 64), shfl_up_dd(x.c1, 1), shfl_up_dd(x.c2, 1),
-                 shfl_up_dd(x.e1, 1), shfl_up_dd(x.q, 1)};
-    if (lane == 0) wex = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    Terms res = terms_add(wex, lds[w]);
+    T wex = tshfl(x, 1);  // wave-exclusive prefix
+    if (lane == 0) wex = T{};
+    T res = tadd(wex, lds[w]);
     total = lds[NT / 64];
     __syncthreads();
     return res;
 }
 
-// One block per row: scan of P (prefix of G), the dd prefix sums C1, C2, E1,
-// Q and PP, sampled at the stored block sizes, then the table entries.
-__global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
-    __shared__ Terms lds[NT / 64 + 1];
+// The G side, one block per (segment s, row b): positions [s seg, min((s + 1)
+// seg, n + 1)), P started at the earlier segments' sum of G.  The segment
+// goes by in chunks of CH positions: the chunk's G is staged in LDS with
+// coalesced loads (one pad word after every run of EK, so that a thread's
+// run is read without bank conflicts; the next chunk's loads are in flight
+// meanwhile); a serial pass over the thread's run, block scans of P and of
+// the run totals, and a second serial pass, only over runs that hold a
+// sampled position, that emits the exclusive prefixes at the samples.
+__global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
+    __shared__ TermsG lds[NT / 64 + 1];
     __shared__ int64_t ldsi[NT / 64 + 1];
-    __shared__ int32_t sG[CH + CH / EK];  // chunk of G, one pad word after every run
-    __shared__ double sA[CH + CH / EK];
-    const uint32_t b = blockIdx.x, n = a.n;
+    __shared__ int32_t sG[CH + CH / EK];
+    const uint32_t s = blockIdx.x, b = blockIdx.y, n = a.n;
     const int32_t* G = a.G + (size_t)b * a.ldg;
-    const double* A = a.A + (size_t)b * n;
     const int64_t c = a.cen[b];
     const int64_t PHn = a.sumg[b] - c * (int64_t)n;
-    __shared__ uint32_t sh_dev;
-    // exactness of the FFT: A is rounded to integers when every value is
-    // within 0.25 of one and A[0] rounds to sum H^2.  The scan assumes it
-    // when A[0] does and checks every value's distance on the way; a row that
-    // fails (not seen in practice) is scanned again without rounding.
-    const dd sh2 = a.sumh2[b];
-    const double A0r = A[0];
-    const double tol = a.round_mode == 2 ? 0.0 : 0.25;
-    bool exact = a.round_mode != 1 && fabs((sh2.hi - rint(A0r)) + sh2.lo) <= 0.5;
-    Terms carry;
-    int64_t pcarry;
-  scan:
-    carry = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    pcarry = 0;
-    double dev = exact ? fabs(A0r - rint(A0r)) : 0.0;
-    // Terms of position i (0..n) given P = P[i] = sum_{j<i} G[j] and av = A[i]
-    // (rounded when exact; 0 outside 1 <= i < n).
-    auto term = [&](uint32_t i, int64_t P, double av) {
-        Terms t = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    const uint64_t base0 = (uint64_t)s * a.seg;
+    const uint64_t end = min(base0 + a.seg, (uint64_t)n + 1);
+    int64_t pcarry = 0;
+    for (uint32_t k = 0; k < s; k++) pcarry += a.gseg[(size_t)b * a.nseg + k];
+    TermsG carry{};
+    auto term = [&](uint32_t i, int64_t P) {
+        TermsG t{};
         t.pp = P;
         const double ph = (double)(P - c * (int64_t)i);
         const double qm = (double)(PHn - (P - c * (int64_t)i));
         if (i < n) t.q = dd_from_prod(qm, qm);
-        if (i >= 1 && i < n) {
-            t.c1 = {av, 0.0};
-            t.c2 = dd_from_prod(av, (double)i);
-            t.e1 = dd_from_prod(ph, ph);
-        }
+        if (i >= 1 && i < n) t.e1 = dd_from_prod(ph, ph);
         return t;
     };
-    // positions 0..n (n+1 values) in chunks of CH: the chunk's G and A are
-    // staged in LDS with coalesced loads (padded so that each thread's run of
-    // EK consecutive positions is read without bank conflicts); then a serial
-    // pass over the thread's run, one block scan of the run totals, and a
-    // second serial pass, only over runs that hold a sampled position, that
-    // emits the exclusive prefixes at the stored block sizes
-    // the next chunk's G and A are loaded into registers while this one is
-    // scanned (CH / NT = EK values of each per thread)
     int32_t pg[EK];
-    double pa[EK];
     auto fetch = [&](uint64_t base) {
 #pragma unroll
         for (uint32_t m = 0; m < EK; m++) {
             const uint64_t i = base + threadIdx.x + m * NT;
             pg[m] = i < n ? G[i] : 0;
-            pa[m] = (i >= 1 && i < n) ? A[i] : 0.0;
         }
     };
-    fetch(0);
-    for (uint64_t base = 0; base <= n; base += CH) {
+    fetch(base0);
+    for (uint64_t base = base0; base < end; base += CH) {
         __syncthreads();  // the previous chunk's LDS reads are done
 #pragma unroll
         for (uint32_t m = 0; m < EK; m++) {
             const uint32_t j = threadIdx.x + m * NT;
             sG[j + j / EK] = pg[m];
-            double av = pa[m];
-            if (exact) {
-                const double ar = rint(av);
-                dev = fmax(dev, fabs(av - ar));
-                av = ar;
-            }
-            sA[j + j / EK] = av;
         }
-        if (base + CH <= n) fetch(base + CH);
+        if (base + CH < end) fetch(base + CH);
         __syncthreads();
         const uint32_t r0 = threadIdx.x * (EK + 1);
         const uint64_t i0 = base + (uint64_t)threadIdx.x * EK;
@@ -838,66 +855,174 @@ __global__ __launch_bounds__(NT) void eval_rows(EvalArgs a) {
         for (uint32_t k = 0; k < EK; k++) gsum += (int64_t)sG[r0 + k];  // 0 past n
         int64_t gtot;
         const int64_t gex = block_incl_scan<int64_t>(gsum, ldsi, gtot) - gsum;
-        Terms loc = {0, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        TermsG loc{};
         int64_t P = pcarry + gex;
         for (uint32_t k = 0; k < EK; k++) {
             const uint64_t i = i0 + k;
-            if (i > n) break;
-            loc = terms_add(loc, term((uint32_t)i, P, sA[r0 + k]));
+            if (i >= end) break;
+            loc = tadd(loc, term((uint32_t)i, P));
             P += (int64_t)sG[r0 + k];
         }
-        Terms tot;
-        Terms run = terms_add(block_excl_scan_terms(loc, lds, tot), carry);
-        if (i0 <= n && a.runflag[i0 / EK]) {
+        TermsG tot;
+        TermsG run = tadd(block_excl_scan_t(loc, lds, tot), carry);
+        if (i0 < end && a.runflag[i0 / EK]) {
             P = pcarry + gex;
             const size_t rb = (size_t)b * a.nl;
             for (uint32_t k = 0; k < EK; k++) {
                 const uint64_t i = i0 + k;
-                if (i > n) break;
+                if (i >= end) break;
                 const int32_t jl = a.pmapL[i], jm = a.pmapM[i];
                 if (jl >= 0) {
                     a.sPP_L[rb + jl] = run.pp;
-                    a.sC1[rb + jl] = run.c1;
-                    a.sC2[rb + jl] = run.c2;
                     a.sE1[rb + jl] = run.e1;
                 }
                 if (jm >= 0) {
                     a.sPP_M[rb + jm] = run.pp;
                     a.sQ[rb + jm] = run.q;
                 }
-                run = terms_add(run, term((uint32_t)i, P, sA[r0 + k]));
+                run = tadd(run, term((uint32_t)i, P));
                 P += (int64_t)sG[r0 + k];
             }
         }
-        carry = terms_add(carry, tot);
+        carry = tadd(carry, tot);
         pcarry += gtot;
     }
-    if (exact) {
-        if (threadIdx.x == 0) sh_dev = 0;
-        __syncthreads();
-        if (dev > tol) sh_dev = 1;
-        __syncthreads();
-        if (sh_dev) {
-            exact = false;
-            goto scan;
+    if (threadIdx.x == 0) a.totg[(size_t)b * a.nseg + s] = carry;
+}
+
+// The A side, one block per (segment s, row b), chunks as in eval_seg_g but
+// loaded at the chunk's start: without registers held for the next chunk the
+// kernel fits 4 blocks per CU, whose loads cover each other's scans.
+// Pass 0 rounds A to integers when row_rounds holds and flags a segment
+// that holds a value further than tol from its integer (not seen in
+// practice); pass 1 scans, unrounded, the rows eval_fix marked for redo.
+__global__ __launch_bounds__(NT) void eval_seg_a(EvalArgs a, uint32_t pass) {
+    __shared__ TermsA lds[NT / 64 + 1];
+    __shared__ double sA[CH + CH / EK];
+    __shared__ uint32_t sh_dev;
+    const uint32_t s = blockIdx.x, b = blockIdx.y, n = a.n;
+    if (pass && !a.redo[b]) return;
+    const double* A = a.A + (size_t)b * n;
+    const bool exact = !pass && row_rounds(a, b);
+    const double tol = a.round_mode == 2 ? 0.0 : 0.25;
+    const uint64_t base0 = (uint64_t)s * a.seg;
+    const uint64_t end = min(base0 + a.seg, (uint64_t)n + 1);
+    TermsA carry{};
+    double dev = 0.0;
+    if (exact && s == 0) dev = fabs(A[0] - rint(A[0]));
+    // A[i] is taken as 0 outside 1 <= i < n, where C1 and C2 gain nothing
+    auto term = [&](uint32_t i, double av) { return TermsA{{av, 0.0}, dd_from_prod(av, (double)i)}; };
+    double pa[EK];
+    auto fetch = [&](uint64_t base) {
+#pragma unroll
+        for (uint32_t m = 0; m < EK; m++) {
+            const uint64_t i = base + threadIdx.x + m * NT;
+            pa[m] = (i >= 1 && i < n) ? A[i] : 0.0;
         }
+    };
+    for (uint64_t base = base0; base < end; base += CH) {
+        fetch(base);
+        __syncthreads();  // the previous chunk's LDS reads are done
+#pragma unroll
+        for (uint32_t m = 0; m < EK; m++) {
+            const uint32_t j = threadIdx.x + m * NT;
+            double av = pa[m];
+            if (exact) {
+                const double ar = rint(av);
+                dev = fmax(dev, fabs(av - ar));
+                av = ar;
+            }
+            sA[j + j / EK] = av;
+        }
+        __syncthreads();
+        const uint32_t r0 = threadIdx.x * (EK + 1);
+        const uint64_t i0 = base + (uint64_t)threadIdx.x * EK;
+        TermsA loc{};
+        for (uint32_t k = 0; k < EK; k++) {
+            const uint64_t i = i0 + k;
+            if (i >= end) break;
+            loc = tadd(loc, term((uint32_t)i, sA[r0 + k]));
+        }
+        TermsA tot;
+        TermsA run = tadd(block_excl_scan_t(loc, lds, tot), carry);
+        if (i0 < end && (a.runflag[i0 / EK] & 1)) {
+            const size_t rb = (size_t)b * a.nl;
+            for (uint32_t k = 0; k < EK; k++) {
+                const uint64_t i = i0 + k;
+                if (i >= end) break;
+                const int32_t jl = a.pmapL[i];
+                if (jl >= 0) {
+                    a.sC1[rb + jl] = run.c1;
+                    a.sC2[rb + jl] = run.c2;
+                }
+                run = tadd(run, term((uint32_t)i, sA[r0 + k]));
+            }
+        }
+        carry = tadd(carry, tot);
     }
-    // carry now holds the totals over positions 0..n: PP[n+1], Q[n+1] = Q[n]
+    if (threadIdx.x == 0) sh_dev = 0;
     __syncthreads();
-    if (threadIdx.x == 0) a.exact[b] = exact ? 1u : 0u;
+    if (dev > tol) sh_dev = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const size_t k = (size_t)b * a.nseg + s;
+        a.tota[k] = carry;
+        a.segdev[k] = exact && sh_dev ? 1u : 0u;
+    }
+}
+
+// One block per row, after eval_seg_g and the same pass of eval_seg_a: the
+// exclusive prefixes of the segments' totals, the samples made row-global,
+// then the table entries.  In pass 0 a row with a flagged segment is marked
+// for redo and left for pass 1; pass 1 handles only the marked rows.
+__global__ __launch_bounds__(NT) void eval_fix(EvalArgs a, uint32_t pass) {
+    __shared__ TermsG preg[MAXSEG + 1];
+    __shared__ TermsA prea[MAXSEG + 1];
+    __shared__ uint32_t sh_redo;
+    const uint32_t b = blockIdx.x, n = a.n;
+    if (pass && !a.redo[b]) return;
+    const bool exact = !pass && row_rounds(a, b);
+    const size_t sb = (size_t)b * a.nseg;
+    if (threadIdx.x == 0) {
+        TermsG rg{};
+        TermsA ra{};
+        uint32_t bad = 0;
+        for (uint32_t k = 0; k < a.nseg; k++) {
+            preg[k] = rg;
+            prea[k] = ra;
+            rg = tadd(rg, a.totg[sb + k]);
+            ra = tadd(ra, a.tota[sb + k]);
+            bad |= a.segdev[sb + k];
+        }
+        preg[a.nseg] = rg;
+        prea[a.nseg] = ra;
+        sh_redo = bad;
+        a.redo[b] = bad;
+        a.exact[b] = exact && !bad ? 1u : 0u;
+    }
+    __syncthreads();
+    if (sh_redo) return;
+    // totals over positions 0..n: PP[n+1], Q[n+1] = Q[n]
+    const double A0r = a.A[(size_t)b * n];
     const double A0 = exact ? rint(A0r) : A0r;
+    const int64_t c = a.cen[b];
     const size_t rb = (size_t)b * a.nl;
-    const int64_t PPend = carry.pp;
-    const dd Qn = carry.q;
+    const int64_t PPend = preg[a.nseg].pp;
+    const dd Qn = preg[a.nseg].q;
     for (uint32_t j = threadIdx.x; j < a.nl; j += NT) {
         const uint64_t L = a.ls[j];
         const uint64_t nw = (uint64_t)n - L + 1;
-        const __int128 S1 = (__int128)PPend - a.sPP_L[rb + j] - a.sPP_M[rb + j];
+        const TermsG& gl = preg[L / a.seg];
+        const TermsG& gm = preg[nw / a.seg];
+        const TermsA& al = prea[L / a.seg];
+        const int64_t ppL = a.sPP_L[rb + j] + gl.pp, ppM = a.sPP_M[rb + j] + gm.pp;
+        const dd C1 = dd_add(a.sC1[rb + j], al.c1), C2 = dd_add(a.sC2[rb + j], al.c2);
+        const dd E1 = dd_add(a.sE1[rb + j], gl.e1), Q = dd_add(a.sQ[rb + j], gm.q);
+        const __int128 S1 = (__int128)PPend - ppL - ppM;
         const __int128 S1H = S1 - (__int128)c * (__int128)L * (__int128)nw;
         // T = L A0 + 2 (L C1 - C2)
-        dd T = dd_add(dd_from_prod(A0, (double)L),
-                      dd_mul_d(dd_add(dd_mul_d(a.sC1[rb + j], (double)L), dd_neg(a.sC2[rb + j])), 2.0));
-        dd E = dd_add(a.sE1[rb + j], dd_add(Qn, dd_neg(a.sQ[rb + j])));
+        dd T = dd_add(dd_from_prod(A0, (double)L), dd_mul_d(dd_add(dd_mul_d(C1, (double)L), dd_neg(C2)), 2.0));
+        dd E = dd_add(E1, dd_add(Qn, dd_neg(Q)));
         dd S2 = dd_add(T, dd_neg(E));
         dd s1h = dd_from_i128(S1H);
         dd num = dd_add(dd_mul_d(S2, (double)nw), dd_neg(dd_mul(s1h, s1h)));
@@ -951,7 +1076,10 @@ void moments_build(Index& ix) {
     DevBuf<uint8_t> runflag;
     DevBuf<int64_t> cen, sumg, out_off, sPP_L, sPP_M;
     DevBuf<dd> sC1, sC2, sE1, sQ, sumh2;
-    DevBuf<uint32_t> d_ls, d_exact;
+    DevBuf<uint32_t> d_ls, d_exact, segdev, redo;
+    DevBuf<int64_t> gseg;
+    DevBuf<TermsG> totg;
+    DevBuf<TermsA> tota;
     DevBuf<double2> X;
     DevBuf<double> A;
     slot_of_fid.reserve(F);
@@ -1003,8 +1131,8 @@ void moments_build(Index& ix) {
         for (uint32_t j = 0; j < nl; j++) {
             pl[ls[j]] = (int32_t)j;
             pm[n - ls[j] + 1] = (int32_t)j;
-            rf[ls[j] / EK] = 1;
-            rf[(n - ls[j] + 1) / EK] = 1;
+            rf[ls[j] / EK] |= 1;
+            rf[(n - ls[j] + 1) / EK] |= 2;
         }
         pmapL.reserve(n + 2);
         pmapM.reserve(n + 2);
@@ -1019,7 +1147,14 @@ void moments_build(Index& ix) {
         // give every CU rows to work on
         size_t free_b = 0, total_b = 0;
         CHK_HIP(hipMemGetInfo(&free_b, &total_b));
+        // evaluation segments: seg positions each (a multiple of CH), at
+        // most MAXSEG per row
+        const uint32_t seg_min = (uint32_t)(((uint64_t)n + 1 + MAXSEG - 1) / MAXSEG);
+        const uint32_t seg = std::max<uint32_t>(ix.tune.eval_seg ? ix.tune.eval_seg : EVAL_SEG,
+                                                (seg_min + CH - 1) / CH * CH);
+        const uint32_t nseg = n / seg + 1;  // positions 0..n
         const uint64_t row_bytes = (uint64_t)n * 12 + 64 * (uint64_t)nl +
+                                   (uint64_t)nseg * (sizeof(TermsG) + sizeof(TermsA) + 12) +
                                    (N <= maxm ? 0 : N * sizeof(double2) / 2);
         const uint32_t B = (uint32_t)std::max<uint64_t>(
             1, std::min<uint64_t>({files.size(), (uint64_t)(free_b / 3) / row_bytes, 4096}));
@@ -1044,6 +1179,11 @@ void moments_build(Index& ix) {
         sC2.reserve((size_t)B * nl);
         sE1.reserve((size_t)B * nl);
         sQ.reserve((size_t)B * nl);
+        gseg.reserve((size_t)B * nseg);
+        totg.reserve((size_t)B * nseg);
+        tota.reserve((size_t)B * nseg);
+        segdev.reserve((size_t)B * nseg);
+        redo.reserve(B);
         const size_t NSUB = (size_t)ix.ngroups * ix.nlayers;  // segments of one shard
         const LayerSeg& Ll = ix.lseg[(size_t)s * NSUB + NSUB - 1];
         const uint64_t soff = ix.lseg[(size_t)s * NSUB].off, scnt = Ll.off + Ll.n - soff;
@@ -1063,7 +1203,7 @@ void moments_build(Index& ix) {
                                    ix.l_end.p + soff, ix.l_fid.p + soff, scnt, slot_of_fid.p, n,
                                    ldg, D.p);
             hipLaunchKernelGGL(coverage_rows, dim3(nb), dim3(NT), 0, st, D.p, ldg, n, cen.p, sumg.p,
-                               sumh2.p);
+                               sumh2.p, seg, nseg, gseg.p);
             CHK_HIP(hipGetLastError());
             Rows r{D.p, ldg, cen.p, X.p, A.p, nb, n, (uint32_t)N, logN, tw.p, twN.p};
             const uint32_t np = (nb + 1) / 2;  // signal pairs
@@ -1096,9 +1236,16 @@ void moments_build(Index& ix) {
             ix.prof.end("moments_fft", t0, st);
             EvalArgs ea{D.p, ldg, n, nl, A.p, cen.p, sumg.p, pmapL.p, pmapM.p, d_ls.p, out_off.p,
                         runflag.p, M.mean.p, M.var.p, sPP_L.p, sPP_M.p, sC1.p, sC2.p, sE1.p, sQ.p, d_exact.p,
-                        sumh2.p, ix.tune.mom_round};
+                        sumh2.p, ix.tune.mom_round, seg, nseg, gseg.p, totg.p, tota.p, segdev.p, redo.p};
             t0 = ix.prof.begin("moments_eval", st);
-            hipLaunchKernelGGL(eval_rows, dim3(nb), dim3(NT), 0, st, ea);
+            // the G side once; the A side in pass 0 over every row, in pass 1
+            // again, unrounded, for the rows whose rounded scan was not exact
+            // (its blocks return at once for the others)
+            hipLaunchKernelGGL(eval_seg_g, dim3(nseg, nb), dim3(NT), 0, st, ea);
+            for (uint32_t pass = 0; pass < 2; pass++) {
+                hipLaunchKernelGGL(eval_seg_a, dim3(nseg, nb), dim3(NT), 0, st, ea, pass);
+                hipLaunchKernelGGL(eval_fix, dim3(nb), dim3(NT), 0, st, ea, pass);
+            }
             CHK_HIP(hipGetLastError());
             ix.prof.end("moments_eval", t0, st);
             std::vector<uint32_t> ex(nb);

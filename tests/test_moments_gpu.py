@@ -147,3 +147,31 @@ def test_rounding_modes_and_rescan(chk, orc):
     assert out[1][1] == 0 and out[2][1] == 0
     for x, y in zip(out[1][2], out[2][2]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_eval_segments_match_oracle(chk, orc, mode):
+    # shard 0: n = 200,000 bins, evaluated in 4 segments by default and in 49
+    # at 4,096 bins; mode 2 scans every row's A side a second time, unrounded,
+    # over all its segments.  Both segmentations match the oracle and each
+    # other (means bit for bit: they are exact integers over the window)
+    rng = np.random.default_rng(57)
+    F, S = 3, 2
+    sizes = np.array([20_000_000, 1_000_000], np.uint64)
+    db = flatten(random_files(rng, F, S, (200, 3000), 20_000_000, 30_000))
+    tabs = []
+    for seg in (0, 4096):
+        ix = chk.Index(*db, S, F)
+        ix.set_genome(sizes)
+        ix.tune("moments.round", mode)
+        ix.tune("moments.eval_seg", seg)
+        assert ix.tune_get("moments.eval_seg") == seg
+        rows, exact = ix.build_moments()
+        assert rows > 0 and exact == (rows if mode == 0 else 0)
+        wm, wv = compare(ix, orc, db, F, S, sizes)
+        assert wm <= REL and wv <= REL, (seg, wm, wv)
+        tabs.append(ix.download_moments())
+    (so0, sl0, m0, v0), (so1, sl1, m1, v1) = tabs
+    assert np.array_equal(so0, so1) and np.array_equal(sl0, sl1)
+    assert np.array_equal(m0, m1)
+    np.testing.assert_allclose(v1, v0, rtol=1e-12, atol=1e-12)
