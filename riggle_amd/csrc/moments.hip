@@ -700,17 +700,18 @@ __global__ __launch_bounds__(NT) void coverage_rows(int32_t* D, uint32_t ldg, ui
 //
 // The table entries need, at the stored block sizes L, prefix sums over the
 // positions i = 0..n of a row: PP (sum of P[i] = sum_{j<i} G[j]), E1 (sum of
-// (P - c i)^2), Q (sum of (PHn - (P - c i))^2), C1 (sum of A[i]) and C2 (sum
-// of A[i] i).  The first three depend on G only and the last two on A only,
-// so two scans of half the width run over the row's segments (seg
+// ph^2, ph = P - c i), Q (sum of (PHn - ph)^2), C1 (sum of A[i]) and C2 (sum
+// of A[i] i).  Q over i < m equals m PHn^2 - 2 PHn (PP - c m (m - 1) / 2) +
+// E1 (ph = 0 at i = 0), so only PP and E1 are scanned on the G side, C1 and
+// C2 on the A side: two scans of half the width run over the row's segments (seg
 // positions each), each segment started from zero sums (and from P at its
 // first position); eval_fix adds the earlier segments' totals.  When A is
 // rounded every term is an integer or an exact product, so the sums do not
 // depend on where the row is cut.
 
-struct TermsG {  // sums of P, (P - c i)^2, (PHn - (P - c i))^2
+struct TermsG {  // sums of P, (P - c i)^2
     int64_t pp;
-    dd e1, q;
+    dd e1;
 };
 struct TermsA {  // sums of A, A i
     dd c1, c2;
@@ -735,7 +736,7 @@ struct EvalArgs {
     dd* sC1;
     dd* sC2;
     dd* sE1;
-    dd* sQ;
+    dd* sE1M;                // E1 sampled at i = n-L+1
     uint32_t* exact;         // [B] 1 when A was rounded to integers
     const dd* sumh2;         // [B] sum H^2, exact
     uint32_t round_mode;     // 0 round within 0.25, 1 never round, 2 round only exact values
@@ -758,13 +759,13 @@ __device__ inline dd shfl_up_dd(dd v, int d) {
     return {__shfl_up(v.hi, d, 64), __shfl_up(v.lo, d, 64)};
 }
 __device__ inline TermsG tadd(const TermsG& a, const TermsG& b) {
-    return {a.pp + b.pp, dd_add(a.e1, b.e1), dd_add(a.q, b.q)};
+    return {a.pp + b.pp, dd_add(a.e1, b.e1)};
 }
 __device__ inline TermsA tadd(const TermsA& a, const TermsA& b) {
     return {dd_add(a.c1, b.c1), dd_add(a.c2, b.c2)};
 }
 __device__ inline TermsG tshfl(const TermsG& v, int d) {
-    return {__shfl_up(v.pp, d, 64), shfl_up_dd(v.e1, d), shfl_up_dd(v.q, d)};
+    return {__shfl_up(v.pp, d, 64), shfl_up_dd(v.e1, d)};
 }
 __device__ inline TermsA tshfl(const TermsA& v, int d) {
     return {shfl_up_dd(v.c1, d), shfl_up_dd(v.c2, d)};
@@ -804,10 +805,11 @@ __device__ T block_excl_scan_t(T v, T* lds, T& total) {
 // seg, n + 1)), P started at the earlier segments' sum of G.  The segment
 // goes by in chunks of CH positions: the chunk's G is staged in LDS with
 // coalesced loads (one pad word after every run of EK, so that a thread's
-// run is read without bank conflicts; the next chunk's loads are in flight
-// meanwhile); a serial pass over the thread's run, block scans of P and of
-// the run totals, and a second serial pass, only over runs that hold a
-// sampled position, that emits the exclusive prefixes at the samples.
+// run is read without bank conflicts); a serial pass over the thread's run,
+// block scans of P and of the run totals, and a second serial pass, only
+// over runs that hold a sampled position, that emits the exclusive prefixes
+// at the samples.  No registers are held for the next chunk's loads: the
+// kernel fits 5 blocks per CU, whose loads cover each other's scans.
 __global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
     __shared__ TermsG lds[NT / 64 + 1];
     __shared__ int64_t ldsi[NT / 64 + 1];
@@ -815,7 +817,6 @@ __global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
     const uint32_t s = blockIdx.x, b = blockIdx.y, n = a.n;
     const int32_t* G = a.G + (size_t)b * a.ldg;
     const int64_t c = a.cen[b];
-    const int64_t PHn = a.sumg[b] - c * (int64_t)n;
     const uint64_t base0 = (uint64_t)s * a.seg;
     const uint64_t end = min(base0 + a.seg, (uint64_t)n + 1);
     int64_t pcarry = 0;
@@ -825,8 +826,6 @@ __global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
         TermsG t{};
         t.pp = P;
         const double ph = (double)(P - c * (int64_t)i);
-        const double qm = (double)(PHn - (P - c * (int64_t)i));
-        if (i < n) t.q = dd_from_prod(qm, qm);
         if (i >= 1 && i < n) t.e1 = dd_from_prod(ph, ph);
         return t;
     };
@@ -838,15 +837,14 @@ __global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
             pg[m] = i < n ? G[i] : 0;
         }
     };
-    fetch(base0);
     for (uint64_t base = base0; base < end; base += CH) {
+        fetch(base);
         __syncthreads();  // the previous chunk's LDS reads are done
 #pragma unroll
         for (uint32_t m = 0; m < EK; m++) {
             const uint32_t j = threadIdx.x + m * NT;
             sG[j + j / EK] = pg[m];
         }
-        if (base + CH < end) fetch(base + CH);
         __syncthreads();
         const uint32_t r0 = threadIdx.x * (EK + 1);
         const uint64_t i0 = base + (uint64_t)threadIdx.x * EK;
@@ -878,7 +876,7 @@ __global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
                 }
                 if (jm >= 0) {
                     a.sPP_M[rb + jm] = run.pp;
-                    a.sQ[rb + jm] = run.q;
+                    a.sE1M[rb + jm] = run.e1;
                 }
                 run = tadd(run, term((uint32_t)i, P));
                 P += (int64_t)sG[r0 + k];
@@ -890,9 +888,8 @@ __global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
     if (threadIdx.x == 0) a.totg[(size_t)b * a.nseg + s] = carry;
 }
 
-// The A side, one block per (segment s, row b), chunks as in eval_seg_g but
-// loaded at the chunk's start: without registers held for the next chunk the
-// kernel fits 4 blocks per CU, whose loads cover each other's scans.
+// The A side, one block per (segment s, row b), chunks as in eval_seg_g (4
+// blocks per CU).
 // Pass 0 rounds A to integers when row_rounds holds and flags a segment
 // that holds a value further than tol from its integer (not seen in
 // practice); pass 1 scans, unrounded, the rows eval_fix marked for redo.
@@ -1002,13 +999,16 @@ __global__ __launch_bounds__(NT) void eval_fix(EvalArgs a, uint32_t pass) {
     }
     __syncthreads();
     if (sh_redo) return;
-    // totals over positions 0..n: PP[n+1], Q[n+1] = Q[n]
+    // totals over positions 0..n: PP[n+1] = PP[n] + P[n] (P[n] = sum G) and
+    // E1[n+1] = E1[n]
     const double A0r = a.A[(size_t)b * n];
     const double A0 = exact ? rint(A0r) : A0r;
     const int64_t c = a.cen[b];
+    const int64_t PHn = a.sumg[b] - c * (int64_t)n;
     const size_t rb = (size_t)b * a.nl;
     const int64_t PPend = preg[a.nseg].pp;
-    const dd Qn = preg[a.nseg].q;
+    const __int128 PPn = (__int128)PPend - a.sumg[b];
+    const dd E1n = preg[a.nseg].e1;
     for (uint32_t j = threadIdx.x; j < a.nl; j += NT) {
         const uint64_t L = a.ls[j];
         const uint64_t nw = (uint64_t)n - L + 1;
@@ -1017,12 +1017,16 @@ __global__ __launch_bounds__(NT) void eval_fix(EvalArgs a, uint32_t pass) {
         const TermsA& al = prea[L / a.seg];
         const int64_t ppL = a.sPP_L[rb + j] + gl.pp, ppM = a.sPP_M[rb + j] + gm.pp;
         const dd C1 = dd_add(a.sC1[rb + j], al.c1), C2 = dd_add(a.sC2[rb + j], al.c2);
-        const dd E1 = dd_add(a.sE1[rb + j], gl.e1), Q = dd_add(a.sQ[rb + j], gm.q);
+        const dd E1 = dd_add(a.sE1[rb + j], gl.e1), E1M = dd_add(a.sE1M[rb + j], gm.e1);
         const __int128 S1 = (__int128)PPend - ppL - ppM;
         const __int128 S1H = S1 - (__int128)c * (__int128)L * (__int128)nw;
         // T = L A0 + 2 (L C1 - C2)
         dd T = dd_add(dd_from_prod(A0, (double)L), dd_mul_d(dd_add(dd_mul_d(C1, (double)L), dd_neg(C2)), 2.0));
-        dd E = dd_add(E1, dd_add(Qn, dd_neg(Q)));
+        // E = E1[L] + Q[n] - Q[M], M = nw: Q[n] - Q[M] = (n - M) PHn^2
+        //   - 2 PHn (PP[n] - PP[M] - c (n (n - 1) - M (M - 1)) / 2) + E1[n] - E1[M]
+        const __int128 dtri = ((__int128)n * (n - 1) - (__int128)nw * (nw - 1)) / 2;
+        const __int128 K = (__int128)(n - nw) * PHn * PHn - 2 * (__int128)PHn * ((PPn - ppM) - (__int128)c * dtri);
+        dd E = dd_add(dd_add(E1, dd_add(E1n, dd_neg(E1M))), dd_from_i128(K));
         dd S2 = dd_add(T, dd_neg(E));
         dd s1h = dd_from_i128(S1H);
         dd num = dd_add(dd_mul_d(S2, (double)nw), dd_neg(dd_mul(s1h, s1h)));
@@ -1075,7 +1079,7 @@ void moments_build(Index& ix) {
     DevBuf<int32_t> slot_of_fid, pmapL, pmapM, D;
     DevBuf<uint8_t> runflag;
     DevBuf<int64_t> cen, sumg, out_off, sPP_L, sPP_M;
-    DevBuf<dd> sC1, sC2, sE1, sQ, sumh2;
+    DevBuf<dd> sC1, sC2, sE1, sE1M, sumh2;
     DevBuf<uint32_t> d_ls, d_exact, segdev, redo;
     DevBuf<int64_t> gseg;
     DevBuf<TermsG> totg;
@@ -1178,7 +1182,7 @@ void moments_build(Index& ix) {
         sC1.reserve((size_t)B * nl);
         sC2.reserve((size_t)B * nl);
         sE1.reserve((size_t)B * nl);
-        sQ.reserve((size_t)B * nl);
+        sE1M.reserve((size_t)B * nl);
         gseg.reserve((size_t)B * nseg);
         totg.reserve((size_t)B * nseg);
         tota.reserve((size_t)B * nseg);
@@ -1235,7 +1239,7 @@ void moments_build(Index& ix) {
             CHK_HIP(hipGetLastError());
             ix.prof.end("moments_fft", t0, st);
             EvalArgs ea{D.p, ldg, n, nl, A.p, cen.p, sumg.p, pmapL.p, pmapM.p, d_ls.p, out_off.p,
-                        runflag.p, M.mean.p, M.var.p, sPP_L.p, sPP_M.p, sC1.p, sC2.p, sE1.p, sQ.p, d_exact.p,
+                        runflag.p, M.mean.p, M.var.p, sPP_L.p, sPP_M.p, sC1.p, sC2.p, sE1.p, sE1M.p, d_exact.p,
                         sumh2.p, ix.tune.mom_round, seg, nseg, gseg.p, totg.p, tota.p, segdev.p, redo.p};
             t0 = ix.prof.begin("moments_eval", st);
             // the G side once; the A side in pass 0 over every row, in pass 1
