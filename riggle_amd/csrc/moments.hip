@@ -821,6 +821,9 @@ __global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
     const uint64_t end = min(base0 + a.seg, (uint64_t)n + 1);
     int64_t pcarry = 0;
     for (uint32_t k = 0; k < s; k++) pcarry += a.gseg[(size_t)b * a.nseg + k];
+    // 0 <= P <= sum G and 0 <= c i <= c n, so |ph| < 2^29 when both bounds
+    // are: a run's sum of 16 squares ph^2 < 2^58 is then exact in int64
+    const bool fast = a.sumg[b] < (1ll << 29) && c * (int64_t)n < (1ll << 29);
     TermsG carry{};
     auto term = [&](uint32_t i, int64_t P) {
         TermsG t{};
@@ -855,11 +858,28 @@ __global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
         const int64_t gex = block_incl_scan<int64_t>(gsum, ldsi, gtot) - gsum;
         TermsG loc{};
         int64_t P = pcarry + gex;
-        for (uint32_t k = 0; k < EK; k++) {
-            const uint64_t i = i0 + k;
-            if (i >= end) break;
-            loc = tadd(loc, term((uint32_t)i, P));
-            P += (int64_t)sG[r0 + k];
+        if (fast) {
+            // run totals in int64: ph = P - c i steps by G - c
+            int64_t spp = 0, se1 = 0, ph = P - c * (int64_t)i0;
+#pragma unroll 8
+            for (uint32_t k = 0; k < EK; k++) {
+                const uint64_t i = i0 + k;
+                if (i >= end) break;
+                spp += P;
+                if (i >= 1 && i < n) se1 += ph * ph;
+                const int64_t g = sG[r0 + k];
+                P += g;
+                ph += g - c;
+            }
+            loc.pp = spp;
+            loc.e1 = dd_from_i128(se1);
+        } else {
+            for (uint32_t k = 0; k < EK; k++) {
+                const uint64_t i = i0 + k;
+                if (i >= end) break;
+                loc = tadd(loc, term((uint32_t)i, P));
+                P += (int64_t)sG[r0 + k];
+            }
         }
         TermsG tot;
         TermsG run = tadd(block_excl_scan_t(loc, lds, tot), carry);
@@ -901,6 +921,9 @@ __global__ __launch_bounds__(NT) void eval_seg_a(EvalArgs a, uint32_t pass) {
     if (pass && !a.redo[b]) return;
     const double* A = a.A + (size_t)b * n;
     const bool exact = !pass && row_rounds(a, b);
+    // rounded, every |A[i]| <= A[0] (an autocorrelation peaks at lag 0); below
+    // 2^45 a run's sums of A and of A k (k < 16) stay below 2^53, exact in FP64
+    const bool fast = exact && rint(A[0]) < 0x1p45;
     const double tol = a.round_mode == 2 ? 0.0 : 0.25;
     const uint64_t base0 = (uint64_t)s * a.seg;
     const uint64_t end = min(base0 + a.seg, (uint64_t)n + 1);
@@ -935,10 +958,23 @@ __global__ __launch_bounds__(NT) void eval_seg_a(EvalArgs a, uint32_t pass) {
         const uint32_t r0 = threadIdx.x * (EK + 1);
         const uint64_t i0 = base + (uint64_t)threadIdx.x * EK;
         TermsA loc{};
-        for (uint32_t k = 0; k < EK; k++) {
-            const uint64_t i = i0 + k;
-            if (i >= end) break;
-            loc = tadd(loc, term((uint32_t)i, sA[r0 + k]));
+        if (fast) {
+            // run totals C1 = sum A, C2 = i0 sum A + sum A k, the sums in FP64
+            double s1 = 0.0, s2 = 0.0;
+#pragma unroll 8
+            for (uint32_t k = 0; k < EK; k++) {  // 0 past n; chunks end at the segment's end
+                const double av = sA[r0 + k];
+                s1 += av;
+                s2 = fma(av, (double)k, s2);
+            }
+            loc.c1 = {s1, 0.0};
+            loc.c2 = dd_add(dd_from_prod((double)i0, s1), dd{s2, 0.0});
+        } else {
+            for (uint32_t k = 0; k < EK; k++) {
+                const uint64_t i = i0 + k;
+                if (i >= end) break;
+                loc = tadd(loc, term((uint32_t)i, sA[r0 + k]));
+            }
         }
         TermsA tot;
         TermsA run = tadd(block_excl_scan_t(loc, lds, tot), carry);

@@ -175,3 +175,22 @@ def test_eval_segments_match_oracle(chk, orc, mode):
     assert np.array_equal(so0, so1) and np.array_equal(sl0, sl1)
     assert np.array_equal(m0, m1)
     np.testing.assert_allclose(v1, v0, rtol=1e-12, atol=1e-12)
+
+
+def test_deep_rounded_rows_match_oracle(chk, orc):
+    # 750 intervals stacked over the first half of a 3 Mb shard: bins of
+    # 75,000 bp coverage against a few hundred, sum H^2 ~ 4.2e13 above 2^45
+    # and sum G ~ 1.1e9 above 2^29, so that row's scans sum their runs in
+    # double-double (the shallow row takes the FP64 and int64 run sums);
+    # both rows stay exact
+    rng = np.random.default_rng(61)
+    F, S = 2, 1
+    sizes = np.array([3_000_000], np.uint64)
+    files = random_files(rng, F, S, (100, 500), 3_000_000, 20_000)
+    deep = (np.zeros(750, np.uint32), np.zeros(750, np.uint32), np.full(750, 1_500_000, np.uint32))
+    files[0] = tuple(np.concatenate([x, y]) for x, y in zip(files[0], deep))
+    db = flatten(files)
+    ix, rows, exact = build(chk, db, F, S, sizes)
+    assert rows == exact == F
+    wm, wv = compare(ix, orc, db, F, S, sizes)
+    assert wm <= REL and wv <= REL, (wm, wv)
