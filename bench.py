@@ -62,7 +62,50 @@ def cpu_baseline(db, q, nshards, nfiles, target_s=4.0, max_threads=16):
                       f"{nq} query intervals vs the full DB; median of 3 runs = {tmed:.3f}s, "
                       f"scaled linearly to the full query ({t_full:.2f}s)"),
            "host_threads_available": os.cpu_count()}
-    return rec, cnt, sel
+    return rec, cnt, sel, L
+
+
+def cpu_batched_baseline(ix, L, batch, swept, F, target_s=4.0, max_threads=16):
+    """The reference CPU path on the C4 batch: the same prefix fraction of
+    every one of the 256 files (so the batch keeps its four 64-file windows,
+    the reference's unit of parallelism), timed on the host threads and
+    scaled to the whole batch; the sample's matrix is checked bit-exact
+    against the GPU's."""
+    sh, s, e, qf, qc = batch
+    B = len(qc)
+    nth = max(1, min(max_threads, os.cpu_count() or 1))
+    off = np.concatenate([[0], np.cumsum(qc)])
+    rank = np.arange(len(qf), dtype=np.int64) - off[qf]
+
+    def pick(phi):
+        keep = rank < np.ceil(phi * qc[qf])
+        return np.nonzero(keep)[0]
+
+    def run(sel):
+        t = time.time()
+        cnt, bins, used = L.query(sh[sel], s[sel], e[sel], qf[sel], B, nthreads=nth)
+        return time.time() - t, cnt, used
+
+    phi = 0.002
+    dt, _, _ = run(pick(phi))
+    phi = min(1.0, phi * target_s / max(dt, 1e-3))
+    sel = pick(phi)
+    frac = len(sel) / len(qf)
+    times = []
+    for _ in range(3):
+        dt, cnt, used = run(sel)
+        times.append(dt)
+    tmed = float(np.median(times))
+    t_full = tmed / frac
+    q = ix.query(sh[sel], s[sel], e[sel], qf[sel], B)
+    gc, _ = ix.sweep(q)
+    return {"value": swept / t_full, "unit": "intervals swept/s", "cores": int(used), "kind": "port",
+            "sample": (f"reference layered CPU sweep (oracle restatement) on the first {frac:.2%} of "
+                       f"every query file's intervals ({len(sel)} of {len(qf)}; 256 files = 4 windows "
+                       f"of 64 => {used} threads as the reference schedules them); median of 3 runs "
+                       f"= {tmed:.3f}s, scaled linearly to the whole batch ({t_full:.1f}s)"),
+            # the reference's u32 counts are the exact counts modulo 2^32
+            "sample_counts_bit_exact": bool(((gc & np.uint64(0xFFFFFFFF)) == cnt.astype(np.uint64)).all())}
 
 
 def profiled_traffic(workload):
@@ -284,6 +327,7 @@ def batched_leg(ix, cfg, F, nshards, D):
     for k in ("sweep.sparse_max", "sweep.items", "sweep.chunk", "sweep.blocks_per_cu"):
         ix.tune(k, 0)
     swept = B * D + int(qc.sum())
+    batched_leg.batch = (sh, s, e, qf, qc)
     # DB bytes: each dense file streams the DB once (g = 1: files do not share
     # staged DB tiles); sparse files read only the layered-table windows they
     # visit, which this figure leaves out
@@ -514,7 +558,7 @@ def main():
         log(f"serving: {rec['serving']}")
 
     if "cpu" in legs:
-        rec_cpu, cnt_cpu, sel = cpu_baseline((sh, s, e, fid), (qsh, qs, qe), nshards, F)
+        rec_cpu, cnt_cpu, sel, L = cpu_baseline((sh, s, e, fid), (qsh, qs, qe), nshards, F)
         # parity on the same sample: GPU counts must equal the reference path's
         qs_ = ix.query(qsh[sel], qs[sel], qe[sel])
         gc, _ = ix.sweep(qs_)
@@ -526,6 +570,13 @@ def main():
                 f"{rec_cpu['moments_and_pvalues']['moments_rel_err_max']}")
         rec["cpu_baseline"] = rec_cpu
         rec["speedup_vs_cpu"] = swept / rec_cpu["value"]
+        if "batched" in rec and hasattr(batched_leg, "batch"):
+            t = time.time()
+            cb = cpu_batched_baseline(ix, L, batched_leg.batch, rec["batched"]["intervals_swept"], F)
+            rec["batched"]["cpu_baseline"] = cb
+            rec["batched"]["speedup_vs_cpu"] = rec["batched"]["batched_swept_per_s"] / cb["value"]
+            log(f"cpu batched baseline {cb['value']:.3e} intervals swept/s on {cb['cores']} threads "
+                f"in {time.time() - t:.1f}s; sample parity {cb['sample_counts_bit_exact']}")
         log(f"cpu baseline {rec_cpu['value']:.3e} intervals/s; sample parity "
             f"{rec_cpu['sample_counts_bit_exact']}")
     print(json.dumps(rec), flush=True)
