@@ -65,6 +65,30 @@ def cpu_baseline(db, q, nshards, nfiles, target_s=4.0, max_threads=16):
     return rec, cnt, sel, L
 
 
+def cpu_serving_baseline(L, files, F, frac=0.05, min_n=2000, max_threads=16):
+    """The reference CPU path on the serving pool: one query_database call per
+    request file, which is one 64-file window and so one thread.  Each
+    file's time comes from its first max(min_n, frac * n) intervals, scaled
+    by size; capacity assumes the host threads each serve one request at a
+    time and scale perfectly (a bound in the CPU's favour)."""
+    nth = max(1, min(max_threads, os.cpu_count() or 1))
+    est = []
+    for sh, s, e in files:
+        n = len(s)
+        k = min(n, max(min_n, int(frac * n)))
+        t = time.time()
+        L.query(sh[:k], s[:k], e[:k], np.zeros(k, np.uint32), 1, nthreads=1)
+        est.append((n, (time.time() - t) * n / k))
+    mean_s = float(np.mean([x[1] for x in est]))
+    est.sort()
+    return {"value": nth / mean_s, "unit": "requests/s", "cores": nth, "kind": "port",
+            "sample": (f"reference layered CPU sweep (oracle restatement), each of the {len(files)} pool "
+                       f"files timed alone on 1 thread over its first max({min_n}, {frac:.0%}) intervals "
+                       f"and scaled by size; capacity = {nth} threads / mean request time "
+                       f"({mean_s * 1e3:.0f} ms), perfect scaling assumed"),
+            "alone_ms_by_size": [(n, round(x * 1e3, 1)) for n, x in est]}
+
+
 def cpu_batched_baseline(ix, L, batch, swept, F, target_s=4.0, max_threads=16):
     """The reference CPU path on the C4 batch: the same prefix fraction of
     every one of the 256 files (so the batch keeps its four 64-file windows,
@@ -366,6 +390,7 @@ def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
              for i, n in enumerate(sizes)]
     log(f"serving: {pool} distinct query files ({int(sizes.sum())} intervals) in "
         f"{time.time() - t:.1f}s")
+    serving_leg.files = files
     srv = riggle_amd.Server(ix, slots=3, max_batch_files=16)
     srv.reserve(int(sizes.sum()))  # any batch of this pool allocates nothing
     # each distinct file alone (latency by size, after one pass over the pool
@@ -577,6 +602,13 @@ def main():
             rec["batched"]["speedup_vs_cpu"] = rec["batched"]["batched_swept_per_s"] / cb["value"]
             log(f"cpu batched baseline {cb['value']:.3e} intervals swept/s on {cb['cores']} threads "
                 f"in {time.time() - t:.1f}s; sample parity {cb['sample_counts_bit_exact']}")
+        if "serving" in rec and hasattr(serving_leg, "files"):
+            t = time.time()
+            cs = cpu_serving_baseline(L, serving_leg.files, F)
+            rec["serving"]["cpu_baseline"] = cs
+            rec["serving"]["capacity_vs_cpu"] = rec["serving"]["capacity_req_per_s"] / cs["value"]
+            log(f"cpu serving baseline {cs['value']:.2f} req/s on {cs['cores']} threads in "
+                f"{time.time() - t:.1f}s")
         log(f"cpu baseline {rec_cpu['value']:.3e} intervals/s; sample parity "
             f"{rec_cpu['sample_counts_bit_exact']}")
     print(json.dumps(rec), flush=True)
