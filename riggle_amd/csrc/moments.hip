@@ -821,8 +821,9 @@ __global__ __launch_bounds__(NT) void eval_seg_g(EvalArgs a) {
     const uint64_t end = min(base0 + a.seg, (uint64_t)n + 1);
     int64_t pcarry = 0;
     for (uint32_t k = 0; k < s; k++) pcarry += a.gseg[(size_t)b * a.nseg + k];
-    // 0 <= P <= sum G and 0 <= c i <= c n, so |ph| < 2^29 when both bounds
-    // are: a run's sum of 16 squares ph^2 < 2^58 is then exact in int64
+    // 0 <= P <= sum G and 0 <= c i <= c n: when sum G and c n are both below
+    // 2^29, |ph| < 2^29 and a run's sum of 16 squares (each < 2^58) is exact
+    // in int64
     const bool fast = a.sumg[b] < (1ll << 29) && c * (int64_t)n < (1ll << 29);
     TermsG carry{};
     auto term = [&](uint32_t i, int64_t P) {
