@@ -194,3 +194,35 @@ def test_deep_rounded_rows_match_oracle(chk, orc):
     assert rows == exact == F
     wm, wv = compare(ix, orc, db, F, S, sizes)
     assert wm <= REL and wv <= REL, (wm, wv)
+
+
+def test_segment_and_chunk_edges_match_oracle(chk, orc):
+    # chromosomes of 2, 3, 4,095, 4,096, 4,097 and 8,192 bins, evaluated in
+    # 4,096-position segments and in the default ones: positions 0..n give a
+    # segment holding only position n (n = 4,096 and 8,192), exactly one
+    # full chunk (n = 4,095), a last chunk of two positions (n = 4,097) and
+    # the two-bin minimum table (L = 1 only)
+    rng = np.random.default_rng(71)
+    nb = [2, 3, 4095, 4096, 4097, 8192]
+    sizes = np.array([100 * k - 50 for k in nb], np.uint64)
+    S, F = len(nb), 3
+    files = []
+    for _ in range(F):
+        sh, s, e = [], [], []
+        for c, size in enumerate(int(x) for x in sizes):
+            k = int(rng.integers(1, 200))
+            st = rng.integers(0, size, k)
+            ln = rng.integers(1, max(2, size // 3), k)
+            sh.append(np.full(k, c, np.uint32))
+            s.append(st.astype(np.uint32))
+            e.append(np.minimum(st + ln, size).astype(np.uint32))
+        files.append((np.concatenate(sh), np.concatenate(s), np.concatenate(e)))
+    db = flatten(files)
+    for seg in (4096, 0):
+        ix = chk.Index(*db, S, F)
+        ix.set_genome(sizes)
+        ix.tune("moments.eval_seg", seg)
+        rows, exact = ix.build_moments()
+        assert rows == exact == S * F
+        wm, wv = compare(ix, orc, db, F, S, sizes)
+        assert wm <= REL and wv <= REL, (seg, wm, wv)
