@@ -40,13 +40,18 @@ def cpu_baseline(db, q, nshards, nfiles, target_s=4.0, max_threads=16):
     L = oracle.LayeredDB(*db, nshards, nfiles, nthreads=nth)
     log(f"cpu baseline: layered index built in {time.time() - t:.1f}s ({L.num_layers} layers)")
     nq = len(q[1])
+
+    def spread(k):  # k query intervals evenly strided over the sorted file: every chromosome
+        return np.unique(np.linspace(0, nq - 1, k).astype(np.int64))
+
     probe = min(nq, 20_000)
-    sel = np.arange(probe)
+    sel = spread(probe)
     t = time.time()
     L.query(q[0][sel], q[1][sel], q[2][sel], np.zeros(probe, np.uint32), 1, nthreads=nth)
     dt = max(time.time() - t, 1e-6)
     n_s = int(min(nq, max(probe, probe * target_s / dt)))
-    sel = np.arange(n_s)
+    sel = spread(n_s)
+    n_s = len(sel)
     times, used = [], 1
     for _ in range(3):
         t = time.time()
@@ -58,8 +63,9 @@ def cpu_baseline(db, q, nshards, nfiles, target_s=4.0, max_threads=16):
     rate = (nq + len(db[1])) / t_full
     rec = {"value": rate, "unit": "intervals/s", "cores": int(used), "kind": "port",
            "sample": (f"reference layered CPU sweep (oracle restatement, 1 query file => 1 window "
-                      f"=> {used} thread as the reference schedules it) on the first {n_s} of "
-                      f"{nq} query intervals vs the full DB; median of 3 runs = {tmed:.3f}s, "
+                      f"=> {used} thread as the reference schedules it) on {n_s} of the "
+                      f"{nq} query intervals, evenly strided over the file, vs the full DB; "
+                      f"median of 3 runs = {tmed:.3f}s, "
                       f"scaled linearly to the full query ({t_full:.2f}s)"),
            "host_threads_available": os.cpu_count()}
     return rec, cnt, sel, L
@@ -68,7 +74,8 @@ def cpu_baseline(db, q, nshards, nfiles, target_s=4.0, max_threads=16):
 def cpu_serving_baseline(L, files, F, frac=0.05, min_n=2000, max_threads=16):
     """The reference CPU path on the serving pool: one query_database call per
     request file, which is one 64-file window and so one thread.  Each
-    file's time comes from its first max(min_n, frac * n) intervals, scaled
+    file's time comes from max(min_n, frac * n) of its intervals, evenly
+    strided over the sorted file, scaled
     by size; capacity assumes the host threads each serve one request at a
     time and scale perfectly (a bound in the CPU's favour)."""
     nth = max(1, min(max_threads, os.cpu_count() or 1))
@@ -76,21 +83,24 @@ def cpu_serving_baseline(L, files, F, frac=0.05, min_n=2000, max_threads=16):
     for sh, s, e in files:
         n = len(s)
         k = min(n, max(min_n, int(frac * n)))
+        idx = np.unique(np.linspace(0, n - 1, k).astype(np.int64))
+        k = len(idx)
         t = time.time()
-        L.query(sh[:k], s[:k], e[:k], np.zeros(k, np.uint32), 1, nthreads=1)
+        L.query(sh[idx], s[idx], e[idx], np.zeros(k, np.uint32), 1, nthreads=1)
         est.append((n, (time.time() - t) * n / k))
     mean_s = float(np.mean([x[1] for x in est]))
     est.sort()
     return {"value": nth / mean_s, "unit": "requests/s", "cores": nth, "kind": "port",
             "sample": (f"reference layered CPU sweep (oracle restatement), each of the {len(files)} pool "
-                       f"files timed alone on 1 thread over its first max({min_n}, {frac:.0%}) intervals "
+                       f"files timed alone on 1 thread over max({min_n}, {frac:.0%}) of its intervals, "
+                       f"evenly strided, "
                        f"and scaled by size; capacity = {nth} threads / mean request time "
                        f"({mean_s * 1e3:.0f} ms), perfect scaling assumed"),
             "alone_ms_by_size": [(n, round(x * 1e3, 1)) for n, x in est]}
 
 
 def cpu_batched_baseline(ix, L, batch, swept, F, target_s=4.0, max_threads=16):
-    """The reference CPU path on the C4 batch: the same prefix fraction of
+    """The reference CPU path on the C4 batch: the same strided fraction of
     every one of the 256 files (so the batch keeps its four 64-file windows,
     the reference's unit of parallelism), timed on the host threads and
     scaled to the whole batch; the sample's matrix is checked bit-exact
@@ -101,9 +111,9 @@ def cpu_batched_baseline(ix, L, batch, swept, F, target_s=4.0, max_threads=16):
     off = np.concatenate([[0], np.cumsum(qc)])
     rank = np.arange(len(qf), dtype=np.int64) - off[qf]
 
-    def pick(phi):
-        keep = rank < np.ceil(phi * qc[qf])
-        return np.nonzero(keep)[0]
+    def pick(phi):  # every (1/phi)-th interval of each sorted file, its first one included
+        step = max(1, int(round(1.0 / phi)))
+        return np.nonzero(rank % step == 0)[0]
 
     def run(sel):
         t = time.time()
@@ -124,8 +134,8 @@ def cpu_batched_baseline(ix, L, batch, swept, F, target_s=4.0, max_threads=16):
     q = ix.query(sh[sel], s[sel], e[sel], qf[sel], B)
     gc, _ = ix.sweep(q)
     return {"value": swept / t_full, "unit": "intervals swept/s", "cores": int(used), "kind": "port",
-            "sample": (f"reference layered CPU sweep (oracle restatement) on the first {frac:.2%} of "
-                       f"every query file's intervals ({len(sel)} of {len(qf)}; 256 files = 4 windows "
+            "sample": (f"reference layered CPU sweep (oracle restatement) on {frac:.2%} of every query "
+                       f"file's intervals, evenly strided ({len(sel)} of {len(qf)}; 256 files = 4 windows "
                        f"of 64 => {used} threads as the reference schedules them); median of 3 runs "
                        f"= {tmed:.3f}s, scaled linearly to the whole batch ({t_full:.1f}s)"),
             # the reference's u32 counts are the exact counts modulo 2^32
@@ -298,7 +308,7 @@ def batched_leg(ix, cfg, F, nshards, D):
     qc = gen.log_uniform_counts(256, 100, 2e6, 0xC40C1E000 + 4, stream=0xC4C4)
     sh, s, e = gen.intervals(g, int(qc.sum()), 0xC40C1E000 + 4, 2, 1000.0, 1.2,
                              hotspot_frac=0.3, hot=hot)
-    qf = np.repeat(np.arange(256, dtype=np.uint32), qc)
+    sh, s, e, qf = gen.sort_files(sh, s, e, np.repeat(np.arange(256, dtype=np.uint32), qc))
     off = np.concatenate([[0], np.cumsum(qc)])
     log(f"batched: generated 256 files, {int(qc.sum())} intervals in {time.time() - t:.1f}s")
     B = 256
@@ -386,7 +396,8 @@ def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
     g = cfg["genome"]
     hot = cfg.get("hot") or gen.hotspots(g, 0xC40C1E000 + 2)
     sizes = gen.log_uniform_counts(pool, 1e3, 5e6, seed, stream=0xC5C5)
-    files = [gen.intervals(g, int(n), seed, 100 + i, 1000.0, 1.2, hotspot_frac=0.4, hot=hot)
+    files = [gen.sort_files(*gen.intervals(g, int(n), seed, 100 + i, 1000.0, 1.2, hotspot_frac=0.4,
+                                           hot=hot))
              for i, n in enumerate(sizes)]
     log(f"serving: {pool} distinct query files ({int(sizes.sum())} intervals) in "
         f"{time.time() - t:.1f}s")

@@ -82,21 +82,35 @@ def log_uniform_counts(nfiles, lo, hi, seed, stream=0xC0C0):
     return np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), nfiles))).astype(np.int64)
 
 
+def sort_files(shard, start, end, fid=None):
+    """Each file's intervals in (chromosome, start) order, the files in fid
+    order (as the generators lay them out)."""
+    f = np.zeros(len(start), np.uint64) if fid is None else fid.astype(np.uint64)
+    key = (f << np.uint64(37)) | (shard.astype(np.uint64) << np.uint64(32)) | start.astype(np.uint64)
+    o = np.argsort(key)
+    if fid is None:
+        return shard[o], start[o], end[o]
+    return shard[o], start[o], end[o], fid[o]
+
+
 def make_db(genome_sizes, counts, seed, hotspot_frac, median, sigma, hot=None):
-    """Files with the given interval counts; returns (shard, start, end, fid)."""
+    """Files with the given interval counts, each sorted by (chromosome,
+    start); returns (shard, start, end, fid)."""
     counts = np.asarray(counts, np.int64)
     N = int(counts.sum())
     shard, start, end = intervals(genome_sizes, N, seed, 1, median, sigma,
                                   hotspot_frac=hotspot_frac, hot=hot)
     fid = np.repeat(np.arange(len(counts), dtype=np.uint32), counts)
-    return shard, start, end, fid
+    return sort_files(shard, start, end, fid)
 
 
 def make_db_chunked(genome_sizes, counts, seed, hotspot_frac, median, sigma, hot=None,
                     chunk=1 << 25, threads=8):
     """make_db for billions of intervals: chunk c is drawn from stream
     (1 << 20) + c, chunks in parallel threads, into preallocated uint32
-    arrays (shard, start, end, fid)."""
+    arrays (shard, start, end, fid).  The files are left unsorted (sorting
+    2e9 intervals on the host would dominate the run; the index build sorts
+    them anyway)."""
     import concurrent.futures as cf
     counts = np.asarray(counts, np.int64)
     N = int(counts.sum())
@@ -124,7 +138,7 @@ def config(name: str, seed: int = 0xC40C1E000):
     if name == "c1":
         g = genome("c1")
         db = make_db(g, np.full(100, 10_000), seed + 1, 0.0, 500.0, 1.0)
-        q = intervals(g, 10_000, seed + 1, 2, 500.0, 1.0)
+        q = sort_files(*intervals(g, 10_000, seed + 1, 2, 500.0, 1.0))
         return {"genome": g, "db": db, "nfiles": 100, "query": q + (np.zeros(10_000, np.uint32),),
                 "nq": 1}
     if name == "c2":
@@ -132,7 +146,7 @@ def config(name: str, seed: int = 0xC40C1E000):
         hot = hotspots(g, seed + 2)
         counts = log_uniform_counts(1000, 1e3, 1e6, seed + 2)
         db = make_db(g, counts, seed + 2, 0.5, 1000.0, 1.2, hot)
-        q = intervals(g, 1_000_000, seed + 2, 2, 1000.0, 1.2, hotspot_frac=0.5, hot=hot)
+        q = sort_files(*intervals(g, 1_000_000, seed + 2, 2, 1000.0, 1.2, hotspot_frac=0.5, hot=hot))
         return {"genome": g, "db": db, "nfiles": 1000,
                 "query": q + (np.zeros(1_000_000, np.uint32),), "nq": 1}
     if name == "c4":
@@ -142,7 +156,7 @@ def config(name: str, seed: int = 0xC40C1E000):
         db = make_db(g, counts, seed + 2, 0.5, 1000.0, 1.2, hot)
         qc = log_uniform_counts(256, 100, 2e6, seed + 4, stream=0xC4C4)
         sh, s, e = intervals(g, int(qc.sum()), seed + 4, 2, 1000.0, 1.2, hotspot_frac=0.3, hot=hot)
-        qf = np.repeat(np.arange(256, dtype=np.uint32), qc)
+        sh, s, e, qf = sort_files(sh, s, e, np.repeat(np.arange(256, dtype=np.uint32), qc))
         return {"genome": g, "db": db, "nfiles": 1000, "query": (sh, s, e, qf), "nq": 256}
     if name == "c5":
         # serving: 10,000 files log-uniform 10k-1M (about 2.15e9 intervals),
@@ -152,7 +166,7 @@ def config(name: str, seed: int = 0xC40C1E000):
         hot = hotspots(g, seed + 5)
         counts = log_uniform_counts(10_000, 1e4, 1e6, seed + 5)
         db = make_db_chunked(g, counts, seed + 5, 0.4, 1000.0, 1.2, hot)
-        q = intervals(g, 1_000_000, seed + 5, 2, 1000.0, 1.2, hotspot_frac=0.4, hot=hot)
+        q = sort_files(*intervals(g, 1_000_000, seed + 5, 2, 1000.0, 1.2, hotspot_frac=0.4, hot=hot))
         return {"genome": g, "db": db, "nfiles": 10_000, "hot": hot,
                 "query": q + (np.zeros(1_000_000, np.uint32),), "nq": 1}
     raise ValueError(name)
