@@ -17,7 +17,8 @@
  *   chk_query_create      -> parse_file_batch's per-shard query intervals
  *                            (src/tasks/query/files.rs:40-111, mod.rs:193-202)
  *   chk_sweep             -> query_sweep_windowed + the window merge
- *                            (src/sweep/query.rs:127-160,
+ *                            (src/sweep/query.rs:84-117, overlap predicate
+ *                             :105-108,
  *                             src/tasks/query/mod.rs:243-298,
  *                             src/matrix/dense.rs:4-160)
  */
@@ -229,6 +230,9 @@ int chk_autotune(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_b
  * (its launch plan -- tiles, chunks, event key widths -- is that batch's); a
  * new batch takes a new query and graph.  It must also be recreated after
  * the index's tuning changes (chk_graph_launch then reports an error).
+ * The graph holds the query handle and the device buffers it was captured
+ * with by address only: the query and every output buffer must outlive the
+ * graph (free the graph first).
  */
 typedef struct chk_graph chk_graph;
 int chk_graph_create(chk_index* idx, chk_query* q, uint64_t* d_counts, uint64_t* d_bp,
@@ -271,7 +275,10 @@ int chk_server_reset_stats(chk_server* s);
  * mid-stream frees the old one, which waits for the whole device and stalls
  * every other slot.  No counterpart in the reference. */
 int chk_server_reserve(chk_server* s, uint64_t max_batch_intervals);
-/* Waits for in-flight batches, then stops the loop. */
+/* Waits for in-flight batches, then stops the loop.  Requests still waiting
+ * for admission are failed ("server stopped ...").  Do not call
+ * chk_server_wait or chk_server_submit on the server while, or after, it is
+ * freed: collect every ticket first. */
 int chk_server_free(chk_server* s);
 
 /* Launch a plain device copy kernel ("calib_copy") that reads and writes

@@ -6,7 +6,7 @@
  *  1. orc_layered_build / orc_layered_query restate the reference CPU path:
  *     size-class layers (src/io/meta.rs:33-81), start-sorted layer buckets
  *     (src/tasks/build.rs:196-211), jump tables (src/io/jump.rs:46-51,
- *     66-113), the per-query look-back scan (src/sweep/query.rs:127-160,
+ *     66-113), the per-query look-back scan (src/sweep/query.rs:84-117,
  *     predicate d.start < q.end && d.end > q.start, u32 count += 1 and f32
  *     bins += overlap/100 per hit), 64-row Q_SID windows run in parallel and
  *     merged serially (src/tasks/query/mod.rs:193-285).  Shards are processed
@@ -235,7 +235,7 @@ static inline size_t jt_lookup(const orc_layer* L, uint32_t target) {
     return (size_t)L->jt[t];
 }
 
-/* query_sweep_windowed (src/sweep/query.rs:127-160). */
+/* query_sweep_windowed (src/sweep/query.rs:84-117). */
 static void sweep_windowed(const orc_layer* L, const uint32_t* qs, const uint32_t* qe,
                            const uint32_t* qsid, size_t nq, uint32_t base, uint32_t F,
                            uint32_t* cnt, float* bins) {

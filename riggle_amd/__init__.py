@@ -357,6 +357,9 @@ class Graph:
         _check(lib().chk_graph_create(index.handle, query.handle, d_counts.ptr, d_bp.ptr,
                                       ptr(d_p), ptr(d_llr), ptr(d_flags), C.byref(h)))
         self._h, self._index, self._query = h, index, query
+        # the captured kernels write these buffers on every launch: keep them
+        # alive for as long as the graph is
+        self._bufs = (d_counts, d_bp, d_p, d_llr, d_flags)
 
     def launch(self):
         _check(lib().chk_graph_launch(self._h))
@@ -365,6 +368,7 @@ class Graph:
         if getattr(self, "_h", None):
             lib().chk_graph_free(self._h)
             self._h = None
+        self._bufs = ()
 
     def __del__(self):
         try:

@@ -34,10 +34,35 @@ double time_sweeps(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps) {
     return (double)ms / reps;
 }
 
+namespace {
+
+double autotune_body(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps, std::string* report);
+
+}  // namespace
+
 double autotune(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps, std::string* report) {
+    // a candidate that throws (allocation failure while re-planning, a plan
+    // limit) leaves the index's tuning and profiling as they were on entry
+    Index& ix = *q.idx;
+    const Tuning saved = ix.tune;
+    const bool prof = ix.prof.on;
+    try {
+        const double ms = autotune_body(q, d_counts, d_bp, reps, report);
+        ix.prof.on = prof;
+        return ms;
+    } catch (...) {
+        ix.tune = saved;
+        ix.tune_version++;
+        ix.prof.on = prof;
+        throw;
+    }
+}
+
+namespace {
+
+double autotune_body(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps, std::string* report) {
     Index& ix = *q.idx;
     reps = reps > 0 ? reps : 3;
-    const bool prof = ix.prof.on;
     ix.prof.on = false;
     Tuning& t = ix.tune;
     char line[160];
@@ -86,8 +111,9 @@ double autotune(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps, std::str
     }
     ix.tune_version++;
     query_plan(q);
-    ix.prof.on = prof;
     return best;
 }
+
+}  // namespace
 
 }  // namespace chk

@@ -16,7 +16,7 @@
 // (key 2*s+1) and an end event (key 2*e) prefixed by (dense file, shard) and
 // sorted.  An end at x sorts before a DB coordinate x (key 2x+1) and a start
 // at x after it, which is exactly the half-open rule d.start < q.end &&
-// d.end > q.start of src/sweep/query.rs:148-151.  Exclusive prefix sums over
+// d.end > q.start of src/sweep/query.rs:105-108.  Exclusive prefix sums over
 // the sorted events give, at any rank p, rS = # starts, rE = p - rS and
 // W = sum(starts) - sum(ends) of the events ranked before p.
 #include <algorithm>

@@ -221,13 +221,25 @@ struct Server {
         } catch (const Error& e) {
             std::lock_guard<std::mutex> g(mu);
             fatal = e.what();
+            cv_done.notify_all();
             return;
         }
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             bool any_busy = false;
             for (auto& s : slots) any_busy |= s->busy;
-            if (stop && !any_busy) break;
+            if (stop && !any_busy) {
+                // requests never admitted fail rather than leave their waiters blocked
+                const auto now = Clock::now();
+                for (Request* r : pending) {
+                    r->error = "server stopped before the request was admitted";
+                    r->t_done = now;
+                    r->done = true;
+                }
+                pending.clear();
+                cv_done.notify_all();
+                break;
+            }
             // completions
             for (auto& s : slots) {
                 if (!s->busy) continue;

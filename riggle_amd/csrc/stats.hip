@@ -304,6 +304,8 @@ void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, doubl
     CHK_REQUIRE(ix.mom.present, CHK_ERR_NOT_FOUND, "the index has no moment tables");
     CHK_REQUIRE(q.has_groups, CHK_ERR_INVALID,
                 "query created before the index had a genome table: no length groups");
+    CHK_REQUIRE((uint64_t)q.nfiles * ix.nfiles < (1ull << 32), CHK_ERR_INVALID,
+                "query files x DB files must stay below 2^32 cells");
     const uint32_t cells = q.nfiles * ix.nfiles;
     StatsArgs a{q.nfiles, ix.nfiles, ix.nshards, d_counts, d_bp, q.g_off.p, q.g_lint.p, q.g_ci.p,
                 q.g_shard.p, q.g_ss.p, q.g_ss2.p, q.g_cnt.p, ix.mom.d_nbins.p, ix.mom.d_slot_off.p,

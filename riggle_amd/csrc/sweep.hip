@@ -1,7 +1,7 @@
 // sweep.hip -- overlap sweep (count + overlap bp, exact u64), dense and sparse.
 //
 // Dense path.  For one query file Q and one DB interval d on a shard (pair-free
-// form of the reference's per-pair scan, src/sweep/query.rs:148-157):
+// form of the reference's per-pair scan, src/sweep/query.rs:84-117):
 //   count(Q, d) = #{q : q.s < d.e} - #{q : q.e <= d.s}
 //   bp(Q, d)    = CQ(d.e) - CQ(d.s),   CQ(x) = sum_q |[q.s, q.e) n [0, x)|
 // With the query's sorted event stream, a DB coordinate x of rank p (events

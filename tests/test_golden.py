@@ -106,3 +106,116 @@ def test_gpu_moments_match_fixture(chk, orc):
             assert var[i] == pytest.approx(v, rel=1e-6, abs=1e-12), (c["name"], L)
             checked += 1
     assert checked > 50
+
+
+GROUPS = load("group_cases.json")
+ANALYTIC = load("analytic_cases.json")
+
+
+@pytest.mark.parametrize("k", range(len(GROUPS)))
+def test_oracle_group_lengths_match_fixture(orc, k):
+    c = GROUPS[k]
+    li, ss, ss2, cnt = orc.group_lengths(c["len_bp"])
+    assert li.tolist() == c["l_int"] and cnt.tolist() == c["count"]
+    np.testing.assert_allclose(ss, c["sum_scale"], rtol=1e-12)
+    np.testing.assert_allclose(ss2, c["sum_scale2"], rtol=1e-12)
+    # every interval lands in one group, and l_int * sum_scale restores sum l
+    assert int(cnt.sum()) == len(c["len_bp"])
+    assert float((li * ss).sum()) == pytest.approx(sum(c["len_bp"]) / 100.0, rel=1e-12)
+
+
+def analytic_oracle_inputs(orc, c):
+    """Groups (with each chromosome's table slot) and tables of one fixture cell."""
+    tables, slot_of = [], {}
+    for ch, t in c["tables"].items():
+        slot_of[ch] = len(tables)
+        tables.append(((c["chroms"][ch] + 99) // 100, np.array(t["mean"]), np.array(t["var"])))
+    parts = [[], [], [], [], []]
+    for ch, lens in c["query"].items():
+        g = orc.group_lengths(lens)
+        for a, v in zip(parts, g + (np.full(len(g[0]), slot_of.get(ch, -1), np.int32),)):
+            a.append(v)
+    return tuple(np.concatenate(a) for a in parts), tables
+
+
+@pytest.mark.parametrize("k", range(len(ANALYTIC)))
+def test_oracle_analytic_stats_match_fixture(orc, k):
+    c = ANALYTIC[k]
+    groups, tables = analytic_oracle_inputs(orc, c)
+    got = orc.analytic_stats(groups, None, tables, float(c["observed"]))
+    assert (got is not None) == c["record"], c["name"]
+    if got is None:
+        return
+    p, llr = got
+    if c["llr"] is None:
+        assert llr is None and p == 1.0
+    else:
+        assert llr == pytest.approx(c["llr"], rel=1e-9, abs=1e-12), c["name"]
+        # A&S 7.1.26 is within 1.5e-7 of erfc: p within 0.75e-7 of the fixture's
+        assert abs(p - c["p"]) < 0.75e-7 + 1e-12, c["name"]
+
+
+def analytic_gpu_layout(c):
+    """DB rows of one fixture cell: intervals [0, take) per query chromosome
+    that cover exactly `observed` bins of the query, whose intervals lie back
+    to back from 0.  None when the cell cannot be laid out (O = 0 or more
+    than the query's length)."""
+    remaining = 100 * c["observed"]
+    db = []
+    for ch, lens in c["query"].items():
+        take = min(sum(lens), remaining)
+        if take > 0:
+            db.append((ch, 0, take))
+        remaining -= take
+    if remaining > 0 or c["observed"] == 0:
+        return None
+    # every DB file also holds the chromosomes of its tables (the file "has"
+    # those tables), far from the query
+    for ch in c["tables"]:
+        db.append((ch, 900_000, 900_100))
+    return db
+
+
+@pytest.mark.gpu
+def test_gpu_analytic_stats_match_fixture(chk):
+    cases = [c for c in ANALYTIC if analytic_gpu_layout(c) is not None]
+    assert len(cases) >= 7
+    names = list(cases[0]["chroms"])
+    S, F = len(names), len(cases)
+    sid = {ch: i for i, ch in enumerate(names)}
+    dsh, dst, den, dfid = [], [], [], []
+    qsh, qst, qen, qfile = [], [], [], []
+    for f, c in enumerate(cases):
+        for ch, a, e in analytic_gpu_layout(c):
+            dsh.append(sid[ch]); dst.append(a); den.append(e); dfid.append(f)
+        for ch, lens in c["query"].items():
+            pos = 0
+            for ln in lens:
+                qsh.append(sid[ch]); qst.append(pos); qen.append(pos + ln); qfile.append(f)
+                pos += ln
+    u = lambda v: np.array(v, np.uint32)
+    ix = chk.Index(u(dsh), u(dst), u(den), u(dfid), S, F)
+    ix.set_genome(np.array([cases[0]["chroms"][n] for n in names], np.uint64))
+    so = np.full(F * S, -1, np.int64)
+    sl = np.zeros(F * S, np.uint32)
+    mean, var = [], []
+    off = 0
+    for f, c in enumerate(cases):
+        for ch, t in c["tables"].items():
+            so[f * S + sid[ch]] = off
+            sl[f * S + sid[ch]] = t["len"]
+            mean += t["mean"]
+            var += t["var"]
+            off += t["len"]
+    ix.upload_moments(so, sl, np.array(mean), np.array(var))
+    out = ix.stats(ix.query(u(qsh), u(qst), u(qen), u(qfile), F))
+    for f, c in enumerate(cases):
+        assert int(out["bp"][f, f]) == 100 * c["observed"], c["name"]
+        assert bool(out["flags"][f, f] & 1) == c["record"], c["name"]
+        if not c["record"]:
+            assert out["p"][f, f] == 1.0 and out["llr"][f, f] == 0.0
+            continue
+        assert bool(out["flags"][f, f] & 2) == (c["llr"] is not None), c["name"]
+        if c["llr"] is not None:
+            assert out["llr"][f, f] == pytest.approx(c["llr"], rel=1e-9, abs=1e-12), c["name"]
+        assert abs(out["p"][f, f] - c["p"]) < 0.75e-7 + 1e-12, c["name"]

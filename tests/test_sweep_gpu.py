@@ -36,7 +36,7 @@ def run_unit(chk, db, q, nq, F):
     return gpu_sweep(chk, db, q, 1, F, nq)
 
 
-# --- src/sweep/query.rs:202-264 ------------------------------------------------
+# --- src/sweep/query.rs:158-221 ------------------------------------------------
 
 def test_basic_overlap(chk):
     c, bp = run_unit(chk, [(0, 50, 150, 0)], [(0, 40, 160, 0)], 1, 1)
@@ -215,9 +215,9 @@ def test_mixed_batch_chooses_per_file(chk, orc):
 
 @pytest.mark.parametrize("F", [1500, 2600])
 def test_many_files_use_file_groups(chk, orc, F):
-    # more DB files than one LDS histogram holds: dense streams split into
-    # groups of 1024 files; the sparse path switches to direct atomics above
-    # 2048 files
+    # more DB files than one LDS histogram holds: the dense streams and the
+    # layered table are split into groups of 1024 files, each with its own
+    # LDS histogram
     rng = np.random.default_rng(24 + F)
     S = 2
     db = flatten(random_files(rng, F, S, (5, 60), 2_000_000, 30_000))
