@@ -86,6 +86,9 @@ def _declare(L):
     L.orc_layered_query.argtypes = [C.c_void_p, sz, u32p, u32p, u32p, u32p, C.c_uint32,
                                     u32p, f32p, C.c_int]
     L.orc_layered_query.restype = C.c_int
+    L.orc_layered_query_exact.argtypes = [C.c_void_p, sz, u32p, u32p, u32p, u32p, C.c_uint32,
+                                          i64p, i64p, C.c_int]
+    L.orc_layered_query_exact.restype = C.c_int
     L.orc_sweep_bruteforce.argtypes = [sz, u32p, u32p, u32p, u32p, C.c_uint32, sz, u32p, u32p,
                                        u32p, u32p, C.c_uint32, i64p, i64p]
     L.orc_sweep_ranks.argtypes = [sz, u32p, u32p, u32p, u32p, C.c_uint32, sz, u32p, u32p, u32p,
@@ -285,6 +288,14 @@ class LayeredDB:
         used = lib().orc_layered_query(self._h, len(start), _u32(shard), _u32(start), _u32(end),
                                        _u32(qsid), nq, cnt, bins, nthreads)
         return cnt, bins, used
+
+    def query_exact(self, shard, start, end, qsid, nq, nthreads=1):
+        """The reference's windowed scan with exact int64 (counts, bp)."""
+        cnt = np.zeros((nq, self.nsources), np.int64)
+        bp = np.zeros((nq, self.nsources), np.int64)
+        lib().orc_layered_query_exact(self._h, len(start), _u32(shard), _u32(start), _u32(end),
+                                      _u32(qsid), nq, cnt, bp, nthreads)
+        return cnt, bp
 
     def __del__(self):
         if getattr(self, "_h", None):

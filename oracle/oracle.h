@@ -83,6 +83,12 @@ int orc_layered_query(const orc_layered_db* db, size_t nquery,
                       const uint32_t* shard, const uint32_t* start,
                       const uint32_t* end, const uint32_t* qsid, uint32_t nq,
                       uint32_t* counts, float* bins, int nthreads);
+/* The same windowed scan with exact int64 counts and overlap bp [nq x F]
+ * (the reference accumulates u32 counts and f32 bins). */
+int orc_layered_query_exact(const orc_layered_db* db, size_t nquery,
+                            const uint32_t* shard, const uint32_t* start,
+                            const uint32_t* end, const uint32_t* qsid, uint32_t nq,
+                            int64_t* counts, int64_t* bp, int nthreads);
 
 /* Exact oracles: int64 count and overlap-bp matrices [nq x F]. */
 void orc_sweep_bruteforce(size_t ndb, const uint32_t* dshard,

@@ -24,6 +24,7 @@
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 /* ---------------------------------------------------------- layer config */
 
@@ -235,10 +236,12 @@ static inline size_t jt_lookup(const orc_layer* L, uint32_t target) {
     return (size_t)L->jt[t];
 }
 
-/* query_sweep_windowed (src/sweep/query.rs:84-117). */
+/* query_sweep_windowed (src/sweep/query.rs:84-117).  With cnt64 set, the
+ * same scan accumulates exact int64 counts and overlap bp instead of the
+ * reference's u32 counts and f32 bins. */
 static void sweep_windowed(const orc_layer* L, const uint32_t* qs, const uint32_t* qe,
                            const uint32_t* qsid, size_t nq, uint32_t base, uint32_t F,
-                           uint32_t* cnt, float* bins) {
+                           uint32_t* cnt, float* bins, int64_t* cnt64, int64_t* bp64) {
     if (nq == 0 || L->n == 0) return;
     for (size_t k = 0; k < nq; k++) {
         uint32_t lo = qs[k] > L->max_size ? qs[k] - L->max_size : 0;
@@ -252,10 +255,15 @@ static void sweep_windowed(const orc_layer* L, const uint32_t* qs, const uint32_
             uint32_t de = L->end[i];
             if (de > qs[k]) {
                 uint32_t d = L->sid[i];
-                cnt[row + d] += 1;
                 uint32_t hi = qe[k] < de ? qe[k] : de;
                 uint32_t lo2 = qs[k] > ds ? qs[k] : ds;
-                bins[row + d] += (float)(hi - lo2) / (float)ORC_BIN_SIZE;
+                if (cnt64) {
+                    cnt64[row + d] += 1;
+                    bp64[row + d] += (int64_t)(hi - lo2);
+                } else {
+                    cnt[row + d] += 1;
+                    bins[row + d] += (float)(hi - lo2) / (float)ORC_BIN_SIZE;
+                }
             }
         }
     }
@@ -269,6 +277,7 @@ typedef struct {
     uint32_t nrows;
     uint32_t* counts;
     float* bins;
+    int64_t *counts64, *bp64; /* exact form when set */
     uint32_t nwin;
     int next;
     pthread_mutex_t mu;
@@ -288,8 +297,11 @@ static size_t lower_sid(const uint32_t* sid, size_t n, uint32_t v) {
 static void* query_worker(void* p) {
     query_ctx* c = (query_ctx*)p;
     uint32_t F = c->db->nsources;
+    const int exact = c->counts64 != NULL;
     uint32_t* sc = (uint32_t*)malloc((size_t)SWEEP_WINDOW_SIZE * F * 4);
     float* sb = (float*)malloc((size_t)SWEEP_WINDOW_SIZE * F * 4);
+    int64_t* xc = exact ? (int64_t*)malloc((size_t)SWEEP_WINDOW_SIZE * F * 8) : NULL;
+    int64_t* xb = exact ? (int64_t*)malloc((size_t)SWEEP_WINDOW_SIZE * F * 8) : NULL;
     for (;;) {
         pthread_mutex_lock(&c->mu);
         int w = c->next++;
@@ -301,27 +313,44 @@ static void* query_worker(void* p) {
         size_t a = lower_sid(c->qsid, c->nq, lo_sid), b = lower_sid(c->qsid, c->nq, hi_sid);
         memset(sc, 0, (size_t)rows * F * 4);
         memset(sb, 0, (size_t)rows * F * 4);
+        if (exact) {
+            memset(xc, 0, (size_t)rows * F * 8);
+            memset(xb, 0, (size_t)rows * F * 8);
+        }
         for (uint32_t l = 0; l < c->db->nlayers; l++) {
             const orc_layer* L = &c->db->L[(size_t)c->shard * c->db->nlayers + l];
-            sweep_windowed(L, c->qs + a, c->qe + a, c->qsid + a, b - a, lo_sid, F, sc, sb);
+            sweep_windowed(L, c->qs + a, c->qe + a, c->qsid + a, b - a, lo_sid, F, sc, sb, xc, xb);
         }
         /* serial merge (mod.rs:281-285): windows own disjoint row blocks */
         for (size_t i = 0; i < (size_t)rows * F; i++) {
-            c->counts[(size_t)lo_sid * F + i] += sc[i];
-            c->bins[(size_t)lo_sid * F + i] += sb[i];
+            if (exact) {
+                c->counts64[(size_t)lo_sid * F + i] += xc[i];
+                c->bp64[(size_t)lo_sid * F + i] += xb[i];
+            } else {
+                c->counts[(size_t)lo_sid * F + i] += sc[i];
+                c->bins[(size_t)lo_sid * F + i] += sb[i];
+            }
         }
     }
     free(sc);
     free(sb);
+    free(xc);
+    free(xb);
     return NULL;
 }
 
-int orc_layered_query(const orc_layered_db* db, size_t nquery, const uint32_t* shard,
-                      const uint32_t* start, const uint32_t* end, const uint32_t* qsid,
-                      uint32_t nq, uint32_t* counts, float* bins, int nthreads) {
+static int layered_query(const orc_layered_db* db, size_t nquery, const uint32_t* shard,
+                         const uint32_t* start, const uint32_t* end, const uint32_t* qsid,
+                         uint32_t nq, uint32_t* counts, float* bins, int64_t* counts64,
+                         int64_t* bp64, int nthreads) {
     uint32_t F = db->nsources;
-    memset(counts, 0, (size_t)nq * F * 4);
-    memset(bins, 0, (size_t)nq * F * 4);
+    if (counts64) {
+        memset(counts64, 0, (size_t)nq * F * 8);
+        memset(bp64, 0, (size_t)nq * F * 8);
+    } else {
+        memset(counts, 0, (size_t)nq * F * 4);
+        memset(bins, 0, (size_t)nq * F * 4);
+    }
     uint32_t nwin = (nq + SWEEP_WINDOW_SIZE - 1) / SWEEP_WINDOW_SIZE;
     if (nthreads < 1) nthreads = 1;
     int used = nthreads < (int)nwin ? nthreads : (int)nwin;
@@ -360,7 +389,8 @@ int orc_layered_query(const orc_layered_db* db, size_t nquery, const uint32_t* s
             qe[k] = end[src];
             qd[k] = qsid[src];
         }
-        query_ctx c = {db, s, qs, qe, qd, n, nq, counts, bins, nwin, 0, PTHREAD_MUTEX_INITIALIZER};
+        query_ctx c = {db, s, qs, qe, qd, n, nq, counts, bins, counts64, bp64, nwin, 0,
+                       PTHREAD_MUTEX_INITIALIZER};
         pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)used);
         for (int t = 0; t < used; t++) pthread_create(&th[t], NULL, query_worker, &c);
         for (int t = 0; t < used; t++) pthread_join(th[t], NULL);
@@ -368,6 +398,20 @@ int orc_layered_query(const orc_layered_db* db, size_t nquery, const uint32_t* s
     }
     free(cnt); free(key); free(tmp); free(idx); free(fill); free(qs); free(qe); free(qd);
     return used;
+}
+
+int orc_layered_query(const orc_layered_db* db, size_t nquery, const uint32_t* shard,
+                      const uint32_t* start, const uint32_t* end, const uint32_t* qsid,
+                      uint32_t nq, uint32_t* counts, float* bins, int nthreads) {
+    return layered_query(db, nquery, shard, start, end, qsid, nq, counts, bins, NULL, NULL,
+                         nthreads);
+}
+
+int orc_layered_query_exact(const orc_layered_db* db, size_t nquery, const uint32_t* shard,
+                            const uint32_t* start, const uint32_t* end, const uint32_t* qsid,
+                            uint32_t nq, int64_t* counts, int64_t* bp, int nthreads) {
+    return layered_query(db, nquery, shard, start, end, qsid, nq, NULL, NULL, counts, bp,
+                         nthreads);
 }
 
 /* ------------------------------------------------------------ exact forms */
@@ -404,6 +448,62 @@ static size_t rank_u32(const uint32_t* v, size_t n, uint32_t x, int le) {
         if (le ? v[m] <= x : v[m] < x) lo = m + 1; else hi = m;
     }
     return lo;
+}
+
+typedef struct {
+    size_t ndb;
+    const uint32_t *dshard, *dstart, *dend, *dsid;
+    uint32_t F, nq, nshards;
+    const size_t* off;
+    const uint32_t *S, *E;
+    const int64_t *PS, *PE;
+    int64_t *counts, *bp;
+    size_t next;
+    pthread_mutex_t mu;
+} ranks_ctx;
+
+#define RANKS_BLOCK ((size_t)1 << 16)
+
+static void* ranks_worker(void* p) {
+    ranks_ctx* c = (ranks_ctx*)p;
+    const uint32_t F = c->F, nq = c->nq;
+    int64_t* cnt = (int64_t*)calloc((size_t)nq * F, 8);
+    int64_t* bp = (int64_t*)calloc((size_t)nq * F, 8);
+    for (;;) {
+        pthread_mutex_lock(&c->mu);
+        size_t i0 = c->next;
+        c->next += RANKS_BLOCK;
+        pthread_mutex_unlock(&c->mu);
+        if (i0 >= c->ndb) break;
+        size_t i1 = i0 + RANKS_BLOCK < c->ndb ? i0 + RANKS_BLOCK : c->ndb;
+        for (size_t i = i0; i < i1; i++) {
+            if (c->dshard[i] >= c->nshards) continue;
+            int64_t x0 = c->dstart[i], x1 = c->dend[i];
+            for (uint32_t q = 0; q < nq; q++) {
+                size_t g = (size_t)c->dshard[i] * nq + q;
+                size_t a = c->off[g], n = c->off[g + 1] - a;
+                if (n == 0) continue;
+                const uint32_t *s = c->S + a, *e = c->E + a;
+                const int64_t *ps = c->PS + a + g, *pe = c->PE + a + g;
+                size_t rs1 = rank_u32(s, n, (uint32_t)x1, 0), re1 = rank_u32(e, n, (uint32_t)x1, 0);
+                size_t rs0 = rank_u32(s, n, (uint32_t)x0, 0), re0 = rank_u32(e, n, (uint32_t)x0, 0);
+                size_t re0le = rank_u32(e, n, (uint32_t)x0, 1);
+                int64_t cq1 = x1 * (int64_t)rs1 - ps[rs1] - x1 * (int64_t)re1 + pe[re1];
+                int64_t cq0 = x0 * (int64_t)rs0 - ps[rs0] - x0 * (int64_t)re0 + pe[re0];
+                cnt[(size_t)q * F + c->dsid[i]] += (int64_t)rs1 - (int64_t)re0le;
+                bp[(size_t)q * F + c->dsid[i]] += cq1 - cq0;
+            }
+        }
+    }
+    pthread_mutex_lock(&c->mu);
+    for (size_t k = 0; k < (size_t)nq * F; k++) {
+        c->counts[k] += cnt[k];
+        c->bp[k] += bp[k];
+    }
+    pthread_mutex_unlock(&c->mu);
+    free(cnt);
+    free(bp);
+    return NULL;
 }
 
 void orc_sweep_ranks(size_t ndb, const uint32_t* dshard, const uint32_t* dstart,
@@ -443,23 +543,16 @@ void orc_sweep_ranks(size_t ndb, const uint32_t* dshard, const uint32_t* dstart,
             PE[a + g + k + 1] = pe;
         }
     }
-    for (size_t i = 0; i < ndb; i++) {
-        if (dshard[i] >= nshards) continue;
-        int64_t x0 = dstart[i], x1 = dend[i];
-        for (uint32_t q = 0; q < nq; q++) {
-            size_t g = (size_t)dshard[i] * nq + q;
-            size_t a = off[g], n = off[g + 1] - a;
-            if (n == 0) continue;
-            const uint32_t *s = S + a, *e = E + a;
-            const int64_t *ps = PS + a + g, *pe = PE + a + g;
-            size_t rs1 = rank_u32(s, n, (uint32_t)x1, 0), re1 = rank_u32(e, n, (uint32_t)x1, 0);
-            size_t rs0 = rank_u32(s, n, (uint32_t)x0, 0), re0 = rank_u32(e, n, (uint32_t)x0, 0);
-            size_t re0le = rank_u32(e, n, (uint32_t)x0, 1);
-            int64_t cq1 = x1 * (int64_t)rs1 - ps[rs1] - x1 * (int64_t)re1 + pe[re1];
-            int64_t cq0 = x0 * (int64_t)rs0 - ps[rs0] - x0 * (int64_t)re0 + pe[re0];
-            counts[(size_t)q * F + dsid[i]] += (int64_t)rs1 - (int64_t)re0le;
-            bp[(size_t)q * F + dsid[i]] += cq1 - cq0;
-        }
-    }
+    ranks_ctx c = {ndb, dshard, dstart, dend, dsid, F, nq, nshards, off, S, E, PS, PE, counts, bp,
+                   0, PTHREAD_MUTEX_INITIALIZER};
+    /* DB elements in blocks over up to 16 threads, each into a private matrix
+     * merged under the lock: integer sums, so the result is independent of
+     * the split */
+    long nt = sysconf(_SC_NPROCESSORS_ONLN);
+    int T = nt < 1 ? 1 : (nt > 16 ? 16 : (int)nt);
+    if ((size_t)T > ndb / 65536 + 1) T = (int)(ndb / 65536 + 1);
+    pthread_t th[16];
+    for (int t = 0; t < T; t++) pthread_create(&th[t], NULL, ranks_worker, &c);
+    for (int t = 0; t < T; t++) pthread_join(th[t], NULL);
     free(off); free(S); free(E); free(fill); free(PS); free(PE);
 }

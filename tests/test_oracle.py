@@ -153,3 +153,20 @@ def test_stdrng_range_i64_bounds_and_spread(orc):
     v = [r.range_i64(-m, m) for _ in range(2000)]
     assert min(v) >= -m and max(v) <= m
     assert min(v) < -m // 2 and max(v) > m // 2 and abs(sum(v) / len(v)) < m / 10
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_layered_exact_scan_matches_bruteforce(orc, seed):
+    # the reference's windowed scan with exact accumulators equals the pair
+    # brute force (and the rank identity) on random multi-file cases
+    from tests.cases import flatten, random_files
+    rng = np.random.default_rng(seed)
+    S, F, nq = 3, 9, 70
+    db = flatten(random_files(rng, F, S, (20, 300), 400_000, 40_000))
+    q = flatten(random_files(rng, nq, S, (0, 40), 400_000, 60_000))
+    L = orc.LayeredDB(*db, S, F)
+    c, b = L.query_exact(*q, nq, nthreads=3)
+    oc, ob = orc.sweep_bruteforce(db, q, F, nq)
+    rc, rb = orc.sweep_ranks(db, q, F, nq, S)
+    assert (c == oc).all() and (b == ob).all()
+    assert (rc == oc).all() and (rb == ob).all()
