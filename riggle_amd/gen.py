@@ -46,13 +46,25 @@ def hotspots(genome_sizes, seed: int, cover=0.05):
     return out
 
 
+def _shard_of(cum, x):
+    """np.searchsorted(cum, x, side="right") for x in [0, 1) through a 2^16-cell
+    lookup table (exact whenever no cell holds two boundaries)."""
+    M = 1 << 16
+    if len(cum) > 1 and np.diff(cum).min() <= 2.0 / M:
+        return np.searchsorted(cum, x, side="right")
+    table = np.searchsorted(cum, np.arange(M, dtype=np.float64) / M, side="right")
+    ext = np.append(cum, np.inf)
+    t = table[(x * M).astype(np.int64)]
+    return t + (ext[t] <= x)
+
+
 def intervals(genome_sizes, n, seed, stream, median=1000.0, sigma=1.2, clip=(1, 100_000),
               hotspot_frac=0.0, hot=None):
     """n intervals: (shard index uint32, start uint32, end uint32)."""
     rng = _rng(seed, stream)
     sizes = np.array([s for _, s in genome_sizes], np.int64)
     p = sizes / sizes.sum()
-    shard = np.searchsorted(np.cumsum(p), rng.random(n), side="right").astype(np.uint32)
+    shard = _shard_of(np.cumsum(p), rng.random(n)).astype(np.uint32)
     shard = np.minimum(shard, len(sizes) - 1)
     ln = np.rint(np.exp(np.log(median) + sigma * rng.standard_normal(n)))
     ln = np.clip(ln, clip[0], clip[1]).astype(np.int64)
@@ -65,13 +77,15 @@ def intervals(genome_sizes, n, seed, stream, median=1000.0, sigma=1.2, clip=(1, 
         hs = shard[idx]
         u = rng.random(len(idx))
         w = rng.random(len(idx))
-        hstart = np.empty(len(idx), np.int64)
-        for c, (ws, width) in enumerate(hot):
-            sel = np.nonzero(hs == c)[0]
-            if len(sel) == 0:
-                continue
-            win = ws[np.minimum((u[sel] * len(ws)).astype(np.int64), len(ws) - 1)]
-            hstart[sel] = win + (w[sel] * width).astype(np.int64)
+        # every chromosome's windows in one array: interval i picks window
+        # floor(u * k_c) of its chromosome and starts floor(w * width_c) into it
+        nwin = np.array([len(ws) for ws, _ in hot], np.int64)
+        woff = np.concatenate([[0], np.cumsum(nwin)[:-1]])
+        width = np.array([wd for _, wd in hot], np.int64)
+        allws = np.concatenate([ws for ws, _ in hot]).astype(np.int64)
+        k = nwin[hs]
+        wi = np.minimum((u * k.astype(np.float64)).astype(np.int64), k - 1)
+        hstart = allws[woff[hs] + wi] + (w * width[hs].astype(np.float64)).astype(np.int64)
         hstart = np.minimum(hstart, csize[idx] - ln[idx])
         start[idx] = hstart
     return shard, start.astype(np.uint32), (start + ln).astype(np.uint32)
@@ -93,15 +107,15 @@ def sort_files(shard, start, end, fid=None):
     return shard[o], start[o], end[o], fid[o]
 
 
-def make_db(genome_sizes, counts, seed, hotspot_frac, median, sigma, hot=None):
+def make_db(genome_sizes, counts, seed, hotspot_frac, median, sigma, hot=None, sort=True):
     """Files with the given interval counts, each sorted by (chromosome,
-    start); returns (shard, start, end, fid)."""
+    start) unless sort is False; returns (shard, start, end, fid)."""
     counts = np.asarray(counts, np.int64)
     N = int(counts.sum())
     shard, start, end = intervals(genome_sizes, N, seed, 1, median, sigma,
                                   hotspot_frac=hotspot_frac, hot=hot)
     fid = np.repeat(np.arange(len(counts), dtype=np.uint32), counts)
-    return sort_files(shard, start, end, fid)
+    return sort_files(shard, start, end, fid) if sort else (shard, start, end, fid)
 
 
 def make_db_chunked(genome_sizes, counts, seed, hotspot_frac, median, sigma, hot=None,
@@ -133,8 +147,11 @@ def make_db_chunked(genome_sizes, counts, seed, hotspot_frac, median, sigma, hot
     return shard, start, end, fid
 
 
-def config(name: str, seed: int = 0xC40C1E000):
-    """Returns a dict with genome, db arrays, query arrays for a benchmark config."""
+def config(name: str, seed: int = 0xC40C1E000, sort: bool = True):
+    """Returns a dict with genome, db arrays, query arrays for a benchmark config.
+    sort=False leaves the C2 and C4 files in generation order (the same
+    intervals; no result depends on the order, and sorting 1.45e8 intervals
+    on the host costs about 15 s)."""
     if name == "c1":
         g = genome("c1")
         db = make_db(g, np.full(100, 10_000), seed + 1, 0.0, 500.0, 1.0)
@@ -145,18 +162,21 @@ def config(name: str, seed: int = 0xC40C1E000):
         g = genome("c2")
         hot = hotspots(g, seed + 2)
         counts = log_uniform_counts(1000, 1e3, 1e6, seed + 2)
-        db = make_db(g, counts, seed + 2, 0.5, 1000.0, 1.2, hot)
-        q = sort_files(*intervals(g, 1_000_000, seed + 2, 2, 1000.0, 1.2, hotspot_frac=0.5, hot=hot))
+        db = make_db(g, counts, seed + 2, 0.5, 1000.0, 1.2, hot, sort=sort)
+        q = intervals(g, 1_000_000, seed + 2, 2, 1000.0, 1.2, hotspot_frac=0.5, hot=hot)
+        q = sort_files(*q) if sort else q
         return {"genome": g, "db": db, "nfiles": 1000,
                 "query": q + (np.zeros(1_000_000, np.uint32),), "nq": 1}
     if name == "c4":
         g = genome("c2")
         hot = hotspots(g, seed + 2)
         counts = log_uniform_counts(1000, 1e3, 1e6, seed + 2)
-        db = make_db(g, counts, seed + 2, 0.5, 1000.0, 1.2, hot)
+        db = make_db(g, counts, seed + 2, 0.5, 1000.0, 1.2, hot, sort=sort)
         qc = log_uniform_counts(256, 100, 2e6, seed + 4, stream=0xC4C4)
         sh, s, e = intervals(g, int(qc.sum()), seed + 4, 2, 1000.0, 1.2, hotspot_frac=0.3, hot=hot)
-        sh, s, e, qf = sort_files(sh, s, e, np.repeat(np.arange(256, dtype=np.uint32), qc))
+        qf = np.repeat(np.arange(256, dtype=np.uint32), qc)
+        if sort:
+            sh, s, e, qf = sort_files(sh, s, e, qf)
         return {"genome": g, "db": db, "nfiles": 1000, "query": (sh, s, e, qf), "nq": 256}
     if name == "c5":
         # serving: 10,000 files log-uniform 10k-1M (about 2.15e9 intervals),
