@@ -281,6 +281,28 @@ int chk_server_reserve(chk_server* s, uint64_t max_batch_intervals);
  * freed: collect every ticket first. */
 int chk_server_free(chk_server* s);
 
+/*
+ * Allocation-free sweep of one query file.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * results as chk_query_create (nfiles = 1) + chk_sweep_async: n intervals in
+ * host arrays (shard ids as in chk_query_create; ids >= nshards are
+ * skipped), outputs d_counts / d_bp = device buffers of nfiles(index) u64
+ * each, overwritten.  The file is swept by the sparse pair kernel with its
+ * intervals and work items in a static device array of
+ * CHK_NOALLOC_SCRATCH_BYTES per device (held until the process ends); no
+ * hipMalloc, hipFree or other allocating call is made.  n above
+ * CHK_NOALLOC_MAX_INTERVALS is rejected with CHK_ERR_INVALID.  Calls on one
+ * device share its scratch array: the library serialises them with a
+ * per-device lock, so overlapping calls from several threads of a process
+ * wait for each other (work a caller enqueues itself cannot touch the
+ * array).  Blocks until the outputs are written.
+ */
+#define CHK_NOALLOC_MAX_INTERVALS (1u << 20)
+#define CHK_NOALLOC_SCRATCH_BYTES (12ull * CHK_NOALLOC_MAX_INTERVALS + 16ull * 65536)
+int chk_sweep_noalloc(chk_index* idx, uint64_t n, const uint32_t* shard, const uint32_t* start,
+                      const uint32_t* end, uint64_t* d_counts, uint64_t* d_bp);
+
 /* Launch a plain device copy kernel ("calib_copy") that reads and writes
  * `bytes` bytes once; used to scale hardware byte counters. */
 int chk_calibrate_copy(chk_index* idx, uint64_t bytes);

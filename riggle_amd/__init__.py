@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("CHUCKLE_HIP_LIB") or os.path.join(HERE, "libchuckle_hip.so")
 
 CHK_OK = 0
+MAX_NOALLOC = 1 << 20  # CHK_NOALLOC_MAX_INTERVALS
 _ERRORS = {1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "I/O error",
            5: "not found", 6: "internal error"}
 
@@ -85,6 +86,7 @@ def _declare(L):
     L.chk_server_reset_stats.argtypes = [vp]
     L.chk_server_reserve.argtypes = [vp, u64]
     L.chk_server_free.argtypes = [vp]
+    L.chk_sweep_noalloc.argtypes = [vp, u64, vp, vp, vp, vp, vp]
 
 
 def _check(code):
@@ -142,6 +144,13 @@ class Index:
         _check(lib().chk_sweep(self._h, query.handle, cnt.ctypes.data_as(C.c_void_p),
                                bp.ctypes.data_as(C.c_void_p)))
         return cnt, bp
+
+    def sweep_noalloc(self, shard, start, end, d_counts, d_bp):
+        """One query file swept without any device allocation (static scratch;
+        at most MAX_NOALLOC intervals) into device buffers of nfiles u64."""
+        shard, start, end = _u32(shard), _u32(start), _u32(end)
+        _check(lib().chk_sweep_noalloc(self._h, len(start), _ptr(shard), _ptr(start), _ptr(end),
+                                       d_counts.ptr, d_bp.ptr))
 
     def sweep_async(self, query, d_counts, d_bp):
         _check(lib().chk_sweep_async(self._h, query.handle, d_counts.ptr, d_bp.ptr))

@@ -581,6 +581,17 @@ int chk_server_free(chk_server* s) {
     });
 }
 
+int chk_sweep_noalloc(chk_index* idx, uint64_t n, const uint32_t* shard, const uint32_t* start,
+                      const uint32_t* end, uint64_t* d_counts, uint64_t* d_bp) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && d_counts && d_bp, CHK_ERR_INVALID, "NULL argument");
+        CHK_REQUIRE(n == 0 || (shard && start && end), CHK_ERR_INVALID,
+                    "interval arrays must be non-NULL");
+        set_device(idx->ix.device);
+        chk::sweep_noalloc(idx->ix, n, shard, start, end, d_counts, d_bp);
+    });
+}
+
 int chk_calibrate_copy(chk_index* idx, uint64_t bytes) {
     return guarded([&] {
         CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");

@@ -277,6 +277,16 @@ void query_plan(Query& q);
 // merge-path partition, dense sweep, sparse sweep.  Enqueued on `st` with
 // sizes known on the host only, so the sequence can be graph-captured.
 void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st);
+// The sparse pair sweep over `nitems` work items of query intervals given by
+// (shard, start, end) device arrays; rows come from the items.  Intervals on
+// shard ids >= the index's nshards are skipped.
+void sparse_launch(Index& ix, const SparseItem* items, uint32_t nitems, const uint32_t* shard,
+                   const uint32_t* start, const uint32_t* end, uint64_t* d_counts, uint64_t* d_bp,
+                   hipStream_t st);
+// Allocation-free sweep of one query file through the sparse pair path with
+// the library's static device scratch array (noalloc.hip).
+void sweep_noalloc(Index& ix, uint64_t n, const uint32_t* shard, const uint32_t* start,
+                   const uint32_t* end, uint64_t* d_counts, uint64_t* d_bp);
 // Fill n u64 with zero from a kernel (graph-safe clearing).
 void zero_u64(uint64_t* p, size_t n, hipStream_t st);
 // zero fills of three buffers (a kernel, not a memset: sequences may be graph-captured)

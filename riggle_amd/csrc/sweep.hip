@@ -524,6 +524,7 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
 
 struct SparseArgs {
     const SparseItem* items;
+    uint32_t nshards;  // intervals on shard ids >= nshards are skipped
     const uint32_t* shard;
     const uint32_t* start;
     const uint32_t* end;
@@ -560,6 +561,7 @@ __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
         const uint32_t qi = u / NL, K = u - qi * NL;
         const uint32_t i = it.begin + qi;
         const uint32_t s = ufl(a.shard[i]);
+        if (s >= a.nshards) continue;
         const LayerSeg L = a.lseg[((size_t)s * a.ngroups + g) * NL + K];
         if (L.n == 0) continue;
         const int32_t qs = (int32_t)ufl(a.start[i]), qe = (int32_t)ufl(a.end[i]);
@@ -748,6 +750,20 @@ void query_plan(Query& q) {
     q.plan_version = ix.tune_version;
 }
 
+void sparse_launch(Index& ix, const SparseItem* items, uint32_t nitems, const uint32_t* shard,
+                   const uint32_t* start, const uint32_t* end, uint64_t* d_counts, uint64_t* d_bp,
+                   hipStream_t st) {
+    if (!nitems) return;
+    SparseArgs sa{items, ix.nshards, shard, start, end, ix.d_lseg.p, ix.ngroups, ix.nlayers,
+                  ix.nfiles, ix.l_start.p, ix.l_end.p, ix.l_fid.p, ix.jt.p,
+                  reinterpret_cast<unsigned long long*>(d_counts),
+                  reinterpret_cast<unsigned long long*>(d_bp)};
+    hipEvent_t t0 = ix.prof.begin("sparse", st);
+    hipLaunchKernelGGL(sparse_kernel, dim3(nitems), dim3(NT), 0, st, sa);
+    CHK_HIP(hipGetLastError());
+    ix.prof.end("sparse", t0, st);
+}
+
 void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
     Index& ix = *q.idx;
     CHK_REQUIRE(q.plan_version == ix.tune_version, CHK_ERR_INTERNAL, "sweep plan is stale");
@@ -768,15 +784,8 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
         sst = q.aux.s;
     }
     if (!q.items_host.empty()) {
-        SparseArgs sa{q.sitems.p, q.shard.p, q.start.p, q.end.p, ix.d_lseg.p, ix.ngroups, ix.nlayers, F,
-                      ix.l_start.p, ix.l_end.p, ix.l_fid.p, ix.jt.p,
-                      reinterpret_cast<unsigned long long*>(d_counts),
-                      reinterpret_cast<unsigned long long*>(d_bp)};
-        const uint32_t ni = (uint32_t)q.items_host.size();
-        hipEvent_t t0 = ix.prof.begin("sparse", sst);
-        hipLaunchKernelGGL(sparse_kernel, dim3(ni), dim3(NT), 0, sst, sa);
-        CHK_HIP(hipGetLastError());
-        ix.prof.end("sparse", t0, sst);
+        sparse_launch(ix, q.sitems.p, (uint32_t)q.items_host.size(), q.shard.p, q.start.p, q.end.p,
+                      d_counts, d_bp, sst);
     }
     if (!q.jobs_host.empty()) {
         hipEvent_t t0 = ix.prof.begin("qprep", st);
