@@ -66,6 +66,40 @@ int chk_index_create(int device, uint32_t nshards, uint32_t nfiles, uint64_t n,
                      const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                      const uint32_t* fid, chk_index** out);
 int chk_index_free(chk_index* idx);
+
+/*
+ * Open a database directory in the reference's format and build its HBM
+ * index (replaces query_database's opening of meta.json and the layer files,
+ * src/tasks/query/mod.rs:123-241, src/io/meta.rs:124-127,
+ * src/io/layer.rs:34-105): DB/meta.json, DB/shards/<shard>/layer_K.{pos,sid}
+ * read over host threads.  Shard ids follow meta.json's "shards" order, file
+ * ids are the D_SIDs.  The genome table is set to hg38 by shard name
+ * (src/fourier/genome.rs:7-34), and DB/momentmap.chk is loaded when it
+ * matches the database (otherwise build the tables with chk_moments_build).
+ */
+int chk_index_open(const char* db_path, int device, chk_index** out);
+/* Names of an opened database's shards and files (the meta.json sid_map),
+ * NUL-terminated into buf (truncated to len - 1 bytes); needed (optional)
+ * receives the full size with the NUL.  CHK_ERR_NOT_FOUND for an index made
+ * by chk_index_create. */
+int chk_index_shard_name(const chk_index* idx, uint32_t shard, char* buf, uint64_t len,
+                         uint64_t* needed);
+int chk_index_file_name(const chk_index* idx, uint32_t fid, char* buf, uint64_t len,
+                        uint64_t* needed);
+
+/*
+ * `chuckle add` (replaces add_to_database, src/tasks/build.rs:86-276): the
+ * BED / BED.gz files of `input` (a file or a directory) are parsed over host
+ * threads in batches of batch_size files (0: all), every batch's intervals
+ * are sorted by (shard, size layer, start) on `device`, and the layer files,
+ * jump tables and meta.json of db_path are written or extended.  D_SIDs
+ * follow the directory's order, as the reference's read_dir does, or the
+ * sorted path order when sorted_sids != 0.  With stats != 0 the moment
+ * tables of every file are built on the device (chromosomes limited to the
+ * nchroms names when nchroms > 0) and saved as DB/momentmap.chk.
+ */
+int chk_db_add(const char* input, const char* db_path, uint64_t batch_size, int stats,
+               const char* const* chroms, uint32_t nchroms, int sorted_sids, int device);
 int chk_index_info(const chk_index* idx, uint32_t* nshards, uint32_t* nfiles, uint64_t* n,
                    uint64_t* device_bytes);
 
@@ -80,6 +114,17 @@ int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t
                      const uint32_t* start, const uint32_t* end, const uint32_t* qfile,
                      chk_query** out);
 int chk_query_free(chk_query* q);
+/*
+ * Parse BED / BED.gz query files over host threads and upload them as one
+ * batch (replaces parse_file_batch, src/tasks/query/files.rs:40-111, with
+ * src/io/parse.rs:53-141 rules) on an index opened with chk_index_open:
+ * chromosome names map to its shards (intervals on other names are
+ * dropped), empty files take no row, and the other files take rows in path
+ * order; row_of_path (optional, npaths entries) receives each path's row or
+ * 0xFFFFFFFF.  When every file is empty, *out is set to NULL.
+ */
+int chk_query_open(chk_index* idx, uint32_t npaths, const char* const* paths, uint32_t* row_of_path,
+                   chk_query** out);
 /*
  * The batch's sweep plan under the index's current tuning: query files and
  * intervals kept (on shards of the DB), and how many files (and their
@@ -155,6 +200,18 @@ int chk_pvalues_async(chk_index* idx, chk_query* q, const uint64_t* d_counts, co
 /* Sweep + p-values into host arrays of nfiles(query) x nfiles(index); blocks. */
 int chk_query_stats(chk_index* idx, chk_query* q, uint64_t* counts, uint64_t* bp, double* p,
                     double* llr, uint8_t* flags);
+
+/*
+ * Write a result matrix as the reference's StatsOutput JSON (replaces the
+ * output step of src/main.rs:146-211, layout of src/stats/mod.rs:31-60):
+ * with stats != 0 one record per cell with flags bit 0, sorted by p
+ * ascending, with observed_bins = bp / 100, p_value and llr (when flags bit
+ * 1); otherwise one record per nonzero count, sorted by count descending.
+ */
+int chk_write_results(const char* path, uint32_t nquery, const char* const* query_names,
+                      uint32_t nfiles, const char* const* db_names, const uint64_t* counts,
+                      const uint64_t* bp, const double* p, const double* llr, const uint8_t* flags,
+                      int stats);
 
 /* Device memory helpers for callers without a device allocator. */
 int chk_dev_alloc(chk_index* idx, uint64_t bytes, void** ptr);

@@ -3,7 +3,10 @@
 // returns a status code.
 #include <cstring>
 #include <new>
+#include <set>
+#include <unordered_map>
 
+#include "dbfile.hpp"
 #include "engine.hpp"
 
 struct chk_index {
@@ -90,18 +93,96 @@ int chk_index_create(int device, uint32_t nshards, uint32_t nfiles, uint64_t n,
         set_device(device);
         auto* h = new chk_index();
         try {
-            h->ix.device = device;
-            h->ix.nshards = nshards;
-            h->ix.nfiles = nfiles;
-            h->ix.n = n;
-            h->ix.ncu = chk::num_cus(device);
-            CHK_HIP(hipStreamCreateWithFlags(&h->ix.stream, hipStreamNonBlocking));
+            chk::index_init(h->ix, device, nshards, nfiles, n);
             chk::index_build(h->ix, shard, start, end, fid);
         } catch (...) {
             delete h;
             throw;
         }
         *out = h;
+    });
+}
+
+int chk_index_open(const char* db_path, int device, chk_index** out) {
+    return guarded([&] {
+        CHK_REQUIRE(db_path && out, CHK_ERR_INVALID, "NULL argument");
+        *out = nullptr;
+        const std::string db(db_path);
+        const chk::db::Meta meta = chk::db::Meta::load(db);
+        const chk::db::Flat f = chk::db::load_intervals(db, meta);
+        for (uint32_t v : f.sid)
+            CHK_REQUIRE(meta.sid_map.count(v), CHK_ERR_IO, "layer file holds a D_SID missing from meta.json");
+        const uint32_t S = (uint32_t)meta.shards.size(), F = std::max<uint32_t>(1, meta.next_sid());
+        set_device(device);
+        auto* h = new chk_index();
+        try {
+            chk::index_init(h->ix, device, S, F, f.start.size());
+            chk::index_build(h->ix, f.shard.data(), f.start.data(), f.end.data(), f.sid.data());
+            h->ix.shard_names = meta.shards;
+            h->ix.file_names.assign(F, std::string());
+            for (auto& kv : meta.sid_map)
+                if (kv.first < F) h->ix.file_names[kv.first] = kv.second;
+            const auto g = chk::db::genome_for(meta.shards, nullptr);
+            chk::set_genome(h->ix, g.data());
+            chk::db::load_moments(db, h->ix, meta);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+static int copy_name(const std::vector<std::string>& names, uint32_t id, char* buf, uint64_t len,
+                     uint64_t* needed, const char* what) {
+    return guarded([&] {
+        CHK_REQUIRE(!names.empty(), CHK_ERR_NOT_FOUND,
+                    std::string("the index has no ") + what + " names (it was not opened from a database)");
+        CHK_REQUIRE(id < names.size(), CHK_ERR_INVALID, std::string(what) + " id out of range");
+        const std::string& n = names[id];
+        if (needed) *needed = n.size() + 1;
+        if (buf && len) {
+            const size_t k = std::min<size_t>(n.size(), (size_t)len - 1);
+            std::memcpy(buf, n.data(), k);
+            buf[k] = 0;
+        }
+    });
+}
+
+int chk_index_shard_name(const chk_index* idx, uint32_t shard, char* buf, uint64_t len,
+                         uint64_t* needed) {
+    if (!idx) return guarded([] { throw chk::Error(CHK_ERR_INVALID, "NULL index"); });
+    return copy_name(idx->ix.shard_names, shard, buf, len, needed, "shard");
+}
+
+int chk_index_file_name(const chk_index* idx, uint32_t fid, char* buf, uint64_t len,
+                        uint64_t* needed) {
+    if (!idx) return guarded([] { throw chk::Error(CHK_ERR_INVALID, "NULL index"); });
+    return copy_name(idx->ix.file_names, fid, buf, len, needed, "file");
+}
+
+int chk_db_add(const char* input, const char* db_path, uint64_t batch_size, int stats,
+               const char* const* chroms, uint32_t nchroms, int sorted_sids, int device) {
+    return guarded([&] {
+        CHK_REQUIRE(input && db_path && (nchroms == 0 || chroms), CHK_ERR_INVALID, "NULL argument");
+        std::set<std::string> wl;
+        for (uint32_t k = 0; k < nchroms; k++) wl.insert(chroms[k]);
+        chk::db::add(input, db_path, batch_size, stats != 0, wl, sorted_sids != 0, device);
+    });
+}
+
+int chk_write_results(const char* path, uint32_t nquery, const char* const* query_names,
+                      uint32_t nfiles, const char* const* db_names, const uint64_t* counts,
+                      const uint64_t* bp, const double* p, const double* llr, const uint8_t* flags,
+                      int stats) {
+    return guarded([&] {
+        CHK_REQUIRE(path && (nquery == 0 || query_names) && (nfiles == 0 || db_names), CHK_ERR_INVALID,
+                    "NULL argument");
+        const size_t cells = (size_t)nquery * nfiles;
+        CHK_REQUIRE(cells == 0 || (counts && (!stats || (bp && p && llr && flags))), CHK_ERR_INVALID,
+                    "NULL result array");
+        std::vector<std::string> q(query_names, query_names + nquery), d(db_names, db_names + nfiles);
+        chk::db::write_results(path, q, d, counts, bp, p, llr, flags, stats != 0);
     });
 }
 
@@ -145,6 +226,61 @@ int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t
             h->q.stream = idx->ix.stream;
             chk::query_upload(h->q, shard, start, end, qfile);
             chk::query_groups(h->q, shard, start, end, qfile);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int chk_query_open(chk_index* idx, uint32_t npaths, const char* const* paths, uint32_t* row_of_path,
+                   chk_query** out) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && out && (npaths == 0 || paths), CHK_ERR_INVALID, "NULL argument");
+        *out = nullptr;
+        chk::Index& ix = idx->ix;
+        CHK_REQUIRE(!ix.shard_names.empty() || ix.nshards == 0, CHK_ERR_INVALID,
+                    "the index has no shard names: open it with chk_index_open");
+        std::vector<std::string> ps(paths, paths + npaths);
+        const auto files = chk::db::parse_beds(ps, chk::db::host_threads());
+        std::unordered_map<std::string, uint32_t> id;
+        for (uint32_t s = 0; s < ix.shard_names.size(); s++) id[ix.shard_names[s]] = s;
+        // empty files take no row; rows follow the path order (files.rs:79-101)
+        uint32_t rows = 0;
+        uint64_t n = 0;
+        for (uint32_t k = 0; k < npaths; k++) {
+            if (row_of_path) row_of_path[k] = files[k].n ? rows : 0xFFFFFFFFu;
+            rows += files[k].n ? 1 : 0;
+            n += files[k].n;
+        }
+        if (rows == 0) return;  // every file empty: no query (*out stays NULL)
+        std::vector<uint32_t> sh(n), st(n), en(n), qf(n);
+        uint64_t i = 0;
+        uint32_t r = 0;
+        for (auto& f : files) {
+            if (!f.n) continue;
+            for (size_t s = 0; s < f.shards.size(); s++) {
+                auto it = id.find(f.shards[s]);
+                const uint32_t sid = it == id.end() ? ix.nshards : it->second;  // absent: dropped
+                for (auto& iv : f.ivs[s]) {
+                    sh[i] = sid;
+                    st[i] = iv.start;
+                    en[i] = iv.end;
+                    qf[i++] = r;
+                }
+            }
+            r++;
+        }
+        set_device(ix.device);
+        auto* h = new chk_query();
+        try {
+            h->q.idx = &ix;
+            h->q.nfiles = rows;
+            h->q.n_in = n;
+            h->q.stream = ix.stream;
+            chk::query_upload(h->q, sh.data(), st.data(), en.data(), qf.data());
+            chk::query_groups(h->q, sh.data(), st.data(), en.data(), qf.data());
         } catch (...) {
             delete h;
             throw;
@@ -206,15 +342,7 @@ int chk_index_set_genome(chk_index* idx, const uint64_t* shard_size) {
     return guarded([&] {
         CHK_REQUIRE(idx && shard_size, CHK_ERR_INVALID, "NULL argument");
         set_device(idx->ix.device);
-        chk::Moments& M = idx->ix.mom;
-        M.nbins.resize(idx->ix.nshards);
-        for (uint32_t s = 0; s < idx->ix.nshards; s++) {
-            CHK_REQUIRE(shard_size[s] <= 0xFFFFFFFFull, CHK_ERR_INVALID, "shard size exceeds u32");
-            M.nbins[s] = (uint32_t)((shard_size[s] + 99) / 100);
-        }
-        M.d_nbins.reserve(idx->ix.nshards);
-        CHK_HIP(hipMemcpy(M.d_nbins.p, M.nbins.data(), idx->ix.nshards * 4, hipMemcpyHostToDevice));
-        M.genome = true;
+        chk::set_genome(idx->ix, shard_size);
     });
 }
 

@@ -111,6 +111,7 @@ struct LayerSeg {
 // int32 end / u32 file id plus a jump table, 12 bytes per interval + tables.
 struct Index {
     Tuning tune;
+    std::vector<std::string> shard_names, file_names;  // set when opened from a DB directory
     uint32_t tune_version = 0;
     Moments mom;
     int device = 0;
@@ -257,8 +258,13 @@ void moments_download(Index& ix, int64_t* slot_off, uint32_t* slot_len, double* 
 void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, double* d_p,
                  double* d_llr, uint8_t* d_flags, hipStream_t st);
 
+// Binds an empty index to a device (its stream, CU count) for n intervals of
+// nfiles files on nshards shards.
+void index_init(Index& ix, int device, uint32_t nshards, uint32_t nfiles, uint64_t n);
 void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                  const uint32_t* fid);
+// Genome table of the statistics: shard_size[s] bp, 0 = not in the table.
+void set_genome(Index& ix, const uint64_t* shard_size);
 void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                   const uint32_t* qfile);
 // Canonical layout (grouped by file) in the staging buffers -> device, then plan.

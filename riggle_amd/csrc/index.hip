@@ -144,6 +144,30 @@ void zero_u64(uint64_t* p, size_t n, hipStream_t st) {
     CHK_HIP(hipGetLastError());
 }
 
+void index_init(Index& ix, int device, uint32_t nshards, uint32_t nfiles, uint64_t n) {
+    CHK_REQUIRE(nfiles > 0, CHK_ERR_INVALID, "nfiles must be positive");
+    CHK_HIP(hipSetDevice(device));
+    ix.device = device;
+    ix.nshards = nshards;
+    ix.nfiles = nfiles;
+    ix.n = n;
+    ix.ncu = num_cus(device);
+    CHK_HIP(hipStreamCreateWithFlags(&ix.stream, hipStreamNonBlocking));
+}
+
+void set_genome(Index& ix, const uint64_t* shard_size) {
+    Moments& M = ix.mom;
+    M.nbins.resize(ix.nshards);
+    for (uint32_t s = 0; s < ix.nshards; s++) {
+        CHK_REQUIRE(shard_size[s] <= 0xFFFFFFFFull, CHK_ERR_INVALID, "shard size exceeds u32");
+        M.nbins[s] = (uint32_t)((shard_size[s] + 99) / 100);
+    }
+    M.d_nbins.reserve(std::max<uint32_t>(1, ix.nshards));
+    if (ix.nshards)
+        CHK_HIP(hipMemcpy(M.d_nbins.p, M.nbins.data(), ix.nshards * 4, hipMemcpyHostToDevice));
+    M.genome = true;
+}
+
 void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                  const uint32_t* fid) {
     const uint64_t n = ix.n;

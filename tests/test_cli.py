@@ -1,8 +1,9 @@
-"""`chuckle add` / `info` on the CPU: the reference's on-disk format
-(meta.json, layer_K.{pos,sid,idx}; src/io/meta.rs, src/io/layer.rs,
-src/io/jump.rs, src/tasks/build.rs:86-276) and BED parsing rules
-(src/io/parse.rs:53-141), checked against the oracle's layer assignment and
-jump tables.  No GPU: `add` without --stats is host work."""
+"""`chuckle add` / `info`: the reference's on-disk format (meta.json,
+layer_K.{pos,sid,idx}; src/io/meta.rs, src/io/layer.rs, src/io/jump.rs,
+src/tasks/build.rs:86-276) and BED parsing rules (src/io/parse.rs:53-141),
+checked against the oracle's layer assignment and jump tables.  `add` sorts
+its buckets on the device, so those tests are marked gpu; parse errors are
+reported before the device is touched and run on the CPU."""
 import gzip
 import json
 import os
@@ -44,6 +45,7 @@ def check_db(orc, db, expected):
     return meta
 
 
+@pytest.mark.gpu
 def test_add_writes_reference_layers(orc, tmp_path):
     src, _ = scenario(orc, str(tmp_path), 4, 700, 0, 0, 5_000_000)
     db = str(tmp_path / "db")
@@ -59,6 +61,7 @@ def test_add_writes_reference_layers(orc, tmp_path):
     check_db(orc, db, expected)
 
 
+@pytest.mark.gpu
 def test_add_appends_and_merges(orc, tmp_path):
     src, _ = scenario(orc, str(tmp_path), 3, 500, 0, 0, 2_000_000)
     db = str(tmp_path / "db")
@@ -79,6 +82,7 @@ def test_add_appends_and_merges(orc, tmp_path):
     check_db(orc, db, expected)
 
 
+@pytest.mark.gpu
 def test_bed_parsing_rules(orc, tmp_path):
     src = tmp_path / "src"
     src.mkdir()
@@ -97,6 +101,25 @@ def test_bed_parsing_rules(orc, tmp_path):
                        ("chrX", 5, 9, a), ("chr1", 100, 200, b)])
 
 
+def test_parse_error_line_number_in_a_large_file(tmp_path):
+    # a file of about 30 MB is parsed in several chunks over host threads:
+    # the error's line number must count every line of the earlier chunks
+    src = tmp_path / "src"
+    src.mkdir()
+    n_before = 1_200_000
+    rng = np.random.default_rng(3)
+    s = rng.integers(0, 10**8, n_before)
+    lines = [f"chr{1 + i % 22}\t{a}\t{a + 100}" for i, a in enumerate(s.tolist())]
+    lines[1000] = "# a comment line"
+    lines[2000] = ""
+    body = "\n".join(lines) + "\nchr1\t500\t400\n" + "chr1\t1\t2\n" * 10
+    (src / "big.bed").write_text(body)
+    assert len(body) > 2**24
+    r = run("add", "-i", str(src), "-d", str(tmp_path / "db"), check=False)
+    assert r.returncode == 1
+    assert r.stderr.strip() == f"Error: Line {n_before + 1}: start (500) must be less than end (400)"
+
+
 def test_bed_errors_exit_1(tmp_path):
     src = tmp_path / "src"
     src.mkdir()
@@ -109,6 +132,7 @@ def test_bed_errors_exit_1(tmp_path):
     assert r.returncode == 1 and "Line 1: invalid format" in r.stderr
 
 
+@pytest.mark.gpu
 def test_info(orc, tmp_path):
     src, _ = scenario(orc, str(tmp_path), 2, 100, 0, 0, 1_000_000)
     db = str(tmp_path / "db")

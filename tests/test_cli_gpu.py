@@ -162,3 +162,57 @@ def test_montecarlo_pvalues_match_oracle(orc, tmp_path, mode):
                 lines.append("")
     assert r.stdout.split("\n")[:len(lines)] == lines
     assert any(obs > 0)
+
+
+def test_index_open_and_query_open_match_oracle(chk, orc, tmp_path):
+    # a CLI-written database opened through the C ABI by path, queried with
+    # files parsed by the library, against the oracle by name
+    import ctypes as C
+    src, qdir = scenario(orc, str(tmp_path), 6, 1500, 3, 800, 20_000_000)
+    (tmp_path / "q" / "empty.bed").write_text("# nothing\n")
+    db = str(tmp_path / "db")
+    run("add", "-i", src, "-d", db, "--sorted-sids")
+    L = chk.lib()
+    L.chk_index_open.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
+    L.chk_index_file_name.argtypes = [C.c_void_p, C.c_uint32, C.c_char_p, C.c_uint64, C.c_void_p]
+    L.chk_index_shard_name.argtypes = [C.c_void_p, C.c_uint32, C.c_char_p, C.c_uint64, C.c_void_p]
+    L.chk_query_open.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_char_p), C.c_void_p,
+                                 C.POINTER(C.c_void_p)]
+    h = C.c_void_p()
+    assert L.chk_index_open(db.encode(), 0, C.byref(h)) == 0, L.chk_last_error()
+    S, F = C.c_uint32(), C.c_uint32()
+    assert L.chk_index_info(h, C.byref(S), C.byref(F), None, None) == 0
+    buf = C.create_string_buffer(256)
+    names = []
+    for f in range(F.value):
+        assert L.chk_index_file_name(h, f, buf, 256, None) == 0
+        names.append(buf.value.decode())
+    assert names == [f"source_{i}.bed" for i in range(6)]  # --sorted-sids: name order
+    shards = []
+    for s_ in range(S.value):
+        assert L.chk_index_shard_name(h, s_, buf, 256, None) == 0
+        shards.append(buf.value.decode())
+    qfiles = sorted(os.path.join(qdir, x) for x in os.listdir(qdir))
+    arr = (C.c_char_p * len(qfiles))(*[x.encode() for x in qfiles])
+    rows = np.zeros(len(qfiles), np.uint32)
+    q = C.c_void_p()
+    assert L.chk_query_open(h, len(qfiles), arr, rows.ctypes.data_as(C.c_void_p), C.byref(q)) == 0, \
+        L.chk_last_error()
+    assert rows[[os.path.basename(x) == "empty.bed" for x in qfiles]].tolist() == [0xFFFFFFFF]
+    nq = int((rows != 0xFFFFFFFF).sum())
+    cnt = np.zeros((nq, F.value), np.uint64)
+    bp = np.zeros((nq, F.value), np.uint64)
+    assert L.chk_sweep(h, q, cnt.ctypes.data_as(C.c_void_p), bp.ctypes.data_as(C.c_void_p)) == 0
+    L.chk_query_free(q)
+    L.chk_index_free(h)
+    for k, path in enumerate(qfiles):
+        if rows[k] == 0xFFFFFFFF:
+            continue
+        qi = int(os.path.basename(path).split("_")[1].split(".")[0])
+        qc, qs, qe = orc.generate_bed(200 + qi, 800, 20_000_000)
+        for f in range(F.value):
+            dc, ds, de = orc.generate_bed(100 + f, 1500, 20_000_000)
+            db_ = (dc.astype(np.uint32), ds, de, np.zeros(len(ds), np.uint32))
+            c, b = orc.sweep_bruteforce(db_, (qc.astype(np.uint32), qs, qe, np.zeros(len(qs), np.uint32)),
+                                        1, 1)
+            assert int(cnt[rows[k], f]) == c[0, 0] and int(bp[rows[k], f]) == b[0, 0]
