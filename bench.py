@@ -61,6 +61,8 @@ def cpu_baseline(db, q, nshards, nfiles, target_s=4.0, max_threads=16):
     tmed = float(np.median(times))
     t_full = tmed * nq / n_s
     rate = (nq + len(db[1])) / t_full
+    # the same scan with exact accumulators: the parity reference for counts and bp
+    xc, xb = L.query_exact(q[0][sel], q[1][sel], q[2][sel], np.zeros(n_s, np.uint32), 1, nthreads=nth)
     rec = {"value": rate, "unit": "intervals/s", "cores": int(used), "kind": "port",
            "sample": (f"reference layered CPU sweep (oracle restatement, 1 query file => 1 window "
                       f"=> {used} thread as the reference schedules it) on {n_s} of the "
@@ -68,7 +70,7 @@ def cpu_baseline(db, q, nshards, nfiles, target_s=4.0, max_threads=16):
                       f"median of 3 runs = {tmed:.3f}s, "
                       f"scaled linearly to the full query ({t_full:.2f}s)"),
            "host_threads_available": os.cpu_count()}
-    return rec, cnt, sel, L
+    return rec, (cnt, xc, xb), sel, L
 
 
 def cpu_serving_baseline(L, files, F, frac=0.05, min_n=2000, max_threads=16):
@@ -132,14 +134,17 @@ def cpu_batched_baseline(ix, L, batch, swept, F, target_s=4.0, max_threads=16):
     tmed = float(np.median(times))
     t_full = tmed / frac
     q = ix.query(sh[sel], s[sel], e[sel], qf[sel], B)
-    gc, _ = ix.sweep(q)
+    gc, gb = ix.sweep(q)
+    xc, xb = L.query_exact(sh[sel], s[sel], e[sel], qf[sel], B, nthreads=nth)
     return {"value": swept / t_full, "unit": "intervals swept/s", "cores": int(used), "kind": "port",
             "sample": (f"reference layered CPU sweep (oracle restatement) on {frac:.2%} of every query "
                        f"file's intervals, evenly strided ({len(sel)} of {len(qf)}; 256 files = 4 windows "
                        f"of 64 => {used} threads as the reference schedules them); median of 3 runs "
                        f"= {tmed:.3f}s, scaled linearly to the whole batch ({t_full:.1f}s)"),
             # the reference's u32 counts are the exact counts modulo 2^32
-            "sample_counts_bit_exact": bool(((gc & np.uint64(0xFFFFFFFF)) == cnt.astype(np.uint64)).all())}
+            "sample_counts_bit_exact": bool(((gc & np.uint64(0xFFFFFFFF)) == cnt.astype(np.uint64)).all()
+                                            and (gc.astype(np.int64) == xc).all()),
+            "sample_bp_bit_exact": bool((gb.astype(np.int64) == xb).all())}
 
 
 def profiled_traffic(workload):
@@ -264,7 +269,7 @@ def cpu_moments_and_precision(ix, cfg, F, stats_out):
         m = qsh == c
         if m.any():
             groups[c] = oracle.group_lengths(qe[m] - qs[m])
-    dmax, checked = 0.0, 0
+    dmax, checked, t_pv = 0.0, 0, 0.0
     for f in range(F):
         if not stats_out["flags"][f] & 1:
             continue
@@ -278,8 +283,10 @@ def cpu_moments_and_precision(ix, cfg, F, stats_out):
                              gvar[o:o + sl[f * S + c]]))
             for a, v in zip(parts, g4 + (np.full(len(g4[0]), slot, np.int32),)):
                 a.append(v)
-        res = oracle.analytic_stats(tuple(np.concatenate(a) for a in parts), None, tabs,
-                                    float(stats_out["bp"][f]) / 100.0)
+        args_ = (tuple(np.concatenate(a) for a in parts), None, tabs, float(stats_out["bp"][f]) / 100.0)
+        t0 = time.perf_counter()
+        res = oracle.analytic_stats(*args_)
+        t_pv += time.perf_counter() - t0
         if res is None:
             continue
         ml = lambda p: np.inf if p <= 0 else -np.log10(p)
@@ -290,6 +297,11 @@ def cpu_moments_and_precision(ix, cfg, F, stats_out):
     return {"moments_rel_err_max": worst, "moment_signals_checked": len(sample),
             "c3_widths_bins": C3_WIDTHS, "pvalue_cells_checked": checked,
             "pvalue_abs_err_mlog10p_max": dmax,
+            "pvalues_cpu": {"value": checked / max(t_pv, 1e-9), "unit": "p-values/s", "cores": 1,
+                            "kind": "port",
+                            "sample": f"the oracle's FP64 compute_analytic_stats on all {checked} emitted "
+                                      f"cells of the query, one thread (timed per call, including the "
+                                      f"ctypes marshalling of each call)"},
             "moments_reference_method_cpu": {"bins_per_s": bins_ref / max(t_ref, 1e-9),
                                              "cores": nth, "kind": "port",
                                              "sample": f"{len(tracks)} (file, chromosome) tracks, "
@@ -385,6 +397,137 @@ def batched_leg(ix, cfg, F, nshards, D):
             "autotuned": knobs, "autotune_s": t_tune, "rows_match_single_launches": rows_ok}
 
 
+def pcie_leg(ix, cfg, F, D, steps):
+    """The plain form with the query handed over in host memory each step:
+    query upload (validation, pinned staging, H2D copy, plan) + sweep + the
+    two result rows copied back.  Reported beside `value`, never as it."""
+    qsh, qs, qe, _ = cfg["query"]
+    dc, db_ = ix.alloc(8 * F), ix.alloc(8 * F)
+    q = ix.query(qsh, qs, qe)
+    for _ in range(2):
+        q.reset(qsh, qs, qe)
+        ix.sweep_async(q, dc, db_)
+        dc.to_numpy(np.uint64, F)
+    t = time.perf_counter()
+    for _ in range(steps):
+        q.reset(qsh, qs, qe)
+        ix.sweep_async(q, dc, db_)
+        dc.to_numpy(np.uint64, F)
+        db_.to_numpy(np.uint64, F)
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    t = time.perf_counter()
+    for _ in range(steps):
+        q2 = ix.query(qsh, qs, qe)
+        ix.sweep_async(q2, dc, db_)
+        dc.to_numpy(np.uint64, F)
+        db_.to_numpy(np.uint64, F)
+        q2.close()
+    ms_new = (time.perf_counter() - t) * 1e3 / steps
+    return {"ms_per_step": ms, "swept_per_s": (len(qs) + D) / (ms * 1e-3), "steps": steps,
+            "region": "host query arrays -> chk_query_reset on a reused handle (host validation, "
+                      "pinned staging, H2D copy, plan) -> sweep -> counts and bp rows copied to the host",
+            "ms_per_step_new_handle": ms_new}
+
+
+def c5_leg(args, parity):
+    """The serving configuration: a 10,000-file DB of about 2.15e9 intervals
+    resident in HBM.  Plain 1M-interval query steps, the Poisson serving mix
+    (>= 1,000 requests of 1k-5M intervals, 40 % hotspot), one end-to-end
+    request from a 5M-interval BED file (parse over host threads, sweep,
+    JSON), and the reference path's CPU serving baseline on a bounded
+    sample."""
+    import riggle_amd
+    from riggle_amd import gen
+    t = time.time()
+    cfg = gen.config("c5")
+    sh, s, e, fid = cfg["db"]
+    S, F, D = len(cfg["genome"]), cfg["nfiles"], len(cfg["db"][1])
+    log(f"c5: generated |D|={D} F={F} in {time.time() - t:.1f}s")
+    t = time.time()
+    ix = riggle_amd.Index(sh, s, e, fid, S, F)
+    t_index = time.time() - t
+    info = ix.info()
+    log(f"c5: index built in {t_index:.1f}s ({info['device_bytes'] / 1e9:.1f} GB)")
+    qsh, qs, qe, _ = cfg["query"]
+    qq = ix.query(qsh, qs, qe)
+    dc, db_ = ix.alloc(8 * F), ix.alloc(8 * F)
+    for _ in range(5):
+        ix.sweep_async(qq, dc, db_)
+    ix.sync()
+    ix.profile(True, only=("sweep",))
+    ix.profile_read("sweep")
+    steps = 20
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ix.sweep_async(qq, dc, db_)
+    ix.sync()
+    ms_step = (time.perf_counter() - t0) * 1e3 / steps
+    n_k, ms_k = ix.profile_read("sweep")
+    ix.profile(False)
+    k_ms = ms_k / max(n_k, 1)
+    alg = 12 * D + 8 * len(qs) + 16 * F
+    out = {"db_files": F, "db_intervals": D, "index_device_bytes": info["device_bytes"],
+           "index_build_s": t_index,
+           "plain": {"query_intervals": len(qs), "ms_per_step": ms_step,
+                     "swept_per_s": (len(qs) + D) / (ms_step * 1e-3), "sweep_kernel_ms": k_ms,
+                     "roofline_frac": alg / (k_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)}}
+    log(f"c5 plain: {ms_step:.2f} ms/step, kernel {k_ms:.2f} ms")
+    serving = serving_leg(ix, cfg, F, D, args.c5_requests, pool=48)
+    out["serving"] = serving
+    log(f"c5 serving: p50 {serving['p50_ms']:.2f} ms p99 {serving['p99_ms']:.2f} ms "
+        f"capacity {serving['capacity_req_per_s']:.1f} req/s")
+    # one request end to end from text: a 5M-interval BED file
+    import tempfile
+    import pandas as pd
+    names = [n for n, _ in cfg["genome"]]
+    ix.set_names(names, [f"db_{k:05d}.bed" for k in range(F)])
+    big = gen.intervals(cfg["genome"], 5_000_000, 0xC40C1E000 + 5, 77, 1000.0, 1.2,
+                        hotspot_frac=0.4, hot=cfg["hot"])
+    tmp = tempfile.mkdtemp(prefix="chk_bench_")
+    bed = os.path.join(tmp, "query_5m.bed")
+    pd.DataFrame({"c": np.array(names)[big[0]], "s": big[1], "e": big[2]}).to_csv(
+        bed, sep="\t", header=False, index=False)
+    js = os.path.join(tmp, "out.json")
+    e2e = None
+    for rep in range(2):  # the second pass is reported (the first warms page caches)
+        t0 = time.perf_counter()
+        q, rows = ix.query_files([bed])
+        t1 = time.perf_counter()
+        ix.sweep_async(q, dc, db_)
+        c = dc.to_numpy(np.uint64, F)
+        b = db_.to_numpy(np.uint64, F)
+        t2 = time.perf_counter()
+        riggle_amd.write_results(js, ["query_5m.bed"], ix.names()[1], c.reshape(1, F), b.reshape(1, F))
+        t3 = time.perf_counter()
+        e2e = {"query_intervals": 5_000_000, "bed_bytes": os.path.getsize(bed),
+               "parse_and_upload_ms": (t1 - t0) * 1e3, "sweep_and_copy_ms": (t2 - t1) * 1e3,
+               "json_ms": (t3 - t2) * 1e3, "json_bytes": os.path.getsize(js),
+               "records": int((c > 0).sum()), "total_ms": (t3 - t0) * 1e3,
+               "host_threads": os.cpu_count()}
+        q.close()
+    # the parsed file gives the same counts as the generated arrays
+    qg = ix.query(*big)
+    ix.sweep_async(qg, dc, db_)
+    parity["c5_parsed_file_rows"] = bool((dc.to_numpy(np.uint64, F) == c).all())
+    out["end_to_end_request"] = e2e
+    log(f"c5 end to end (5M-interval BED): {e2e}")
+    for f in (bed, js):
+        os.remove(f)
+    os.rmdir(tmp)
+    if "cpu" in args.legs_set:
+        import oracle
+        t = time.time()
+        L = oracle.LayeredDB(sh, s, e, fid, S, F, nthreads=16)
+        log(f"c5 cpu: layered index built in {time.time() - t:.1f}s")
+        cs = cpu_serving_baseline(L, serving_leg.files[:16], F, frac=0.01, min_n=1000)
+        out["serving"]["cpu_baseline"] = cs
+        out["serving"]["capacity_vs_cpu"] = serving["capacity_req_per_s"] / cs["value"]
+        log(f"c5 cpu serving baseline {cs['value']:.3f} req/s")
+        del L
+    ix.close()
+    return out
+
+
 def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
     """Serving form against the resident DB: query files of log-uniform 1k-5M
     intervals (40 % hotspot) arrive as a Poisson stream at 65 % of the
@@ -403,7 +546,8 @@ def serving_leg(ix, cfg, F, D, n_requests, seed=0xC40C1E000 + 5, pool=32):
         f"{time.time() - t:.1f}s")
     serving_leg.files = files
     srv = riggle_amd.Server(ix, slots=3, max_batch_files=16)
-    srv.reserve(int(sizes.sum()))  # any batch of this pool allocates nothing
+    # any batch of this pool (at most 16 files) allocates nothing
+    srv.reserve(int(np.sort(sizes)[-16:].sum()))
     # each distinct file alone (latency by size, after one pass over the pool
     # has grown the slots' buffers), then capacity: closed loop over the pool,
     # twice (the first pass warms up)
@@ -475,8 +619,10 @@ def main():
                     help="launch-shape knob key=value (see chk_tune), may repeat")
     ap.add_argument("--skip-stats", action="store_true", help="skip moments + p-value legs")
     ap.add_argument("--skip-batched", action="store_true", help="skip the batched 256-file leg")
-    ap.add_argument("--serve-requests", type=int, default=300,
+    ap.add_argument("--serve-requests", type=int, default=1000,
                     help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--c5-requests", type=int, default=1000,
+                    help="requests of the serving leg on the 10,000-file C5 DB")
     ap.add_argument("--legs", default=None,
                     help="comma list of legs to run (plain,stats,batched,serving,cpu); the "
                          "plain leg gives the JSON line's value")
@@ -484,8 +630,11 @@ def main():
                     help="also run the plain copy kernel over this many bytes (counter scaling)")
     args = ap.parse_args()
     if args.legs is None:  # the serving DB is too large for the CPU and moment legs
-        args.legs = "plain,serving" if args.config == "c5" else "plain,stats,batched,serving,cpu"
+        args.legs = ("plain,serving" if args.config == "c5"
+                     else "plain,pcie,stats,batched,serving,cpu,c5")
     legs = set(args.legs.split(","))
+    if args.config != "c2":
+        legs.discard("c5")
     if args.no_cpu:
         legs.discard("cpu")
     if args.skip_stats:
@@ -494,6 +643,8 @@ def main():
         legs.discard("batched")
     if args.serve_requests <= 0:
         legs.discard("serving")
+    args.legs_set = legs
+    parity = {}
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -571,6 +722,9 @@ def main():
         "dtype": "int32 compare, u64/int64 accumulate", "data": "synthetic",
         "config": {"workload": f"{args.config} plain: 1 query file vs {F}-file DB",
                    "db_intervals": D, "query_intervals": Q, "db_files": F,
+                   "timed_region": "query resident in HBM: event sort + prefix sums, merge-path "
+                                   "partition, sweep kernel, exact u64 count and bp rows "
+                                   "(PCIe-inclusive rate under pcie_inclusive)",
                    "genome_bp": int(sum(x for _, x in cfg["genome"])), "chromosomes": nshards},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -585,23 +739,35 @@ def main():
     if calib:
         rec["calibration_copy"] = calib
 
+    if "pcie" in legs:
+        rec["pcie_inclusive"] = pcie_leg(ix, cfg, F, D, 10)
+        log(f"pcie-inclusive: {rec['pcie_inclusive']['ms_per_step']:.3f} ms/step")
     if "stats" in legs:
         rec["stats"] = stats_leg(ix, qq, cfg, F, args.steps)
     if "batched" in legs and args.config == "c2":
         rec["batched"] = batched_leg(ix, cfg, F, nshards, D)
+        parity["batched_rows_match_single_launches"] = rec["batched"]["rows_match_single_launches"]
     if "serving" in legs and args.config in ("c2", "c5"):
         rec["serving"] = serving_leg(ix, cfg, F, D, args.serve_requests)
         log(f"serving: {rec['serving']}")
 
     if "cpu" in legs:
-        rec_cpu, cnt_cpu, sel, L = cpu_baseline((sh, s, e, fid), (qsh, qs, qe), nshards, F)
-        # parity on the same sample: GPU counts must equal the reference path's
+        rec_cpu, (cnt_cpu, xc, xb), sel, L = cpu_baseline((sh, s, e, fid), (qsh, qs, qe), nshards, F)
+        # parity on the same sample: GPU counts and bp must equal the reference
+        # scan's (exact accumulators; its u32 counts agree modulo 2^32)
         qs_ = ix.query(qsh[sel], qs[sel], qe[sel])
-        gc, _ = ix.sweep(qs_)
-        rec_cpu["sample_counts_bit_exact"] = bool((gc[0] == cnt_cpu[0].astype(np.uint64)).all())
+        gc, gb = ix.sweep(qs_)
+        rec_cpu["sample_counts_bit_exact"] = bool((gc[0].astype(np.int64) == xc[0]).all() and
+                                                  ((gc[0] & np.uint64(0xFFFFFFFF)) ==
+                                                   cnt_cpu[0].astype(np.uint64)).all())
+        rec_cpu["sample_bp_bit_exact"] = bool((gb[0].astype(np.int64) == xb[0]).all())
+        parity["plain_sample_counts"] = rec_cpu["sample_counts_bit_exact"]
+        parity["plain_sample_bp"] = rec_cpu["sample_bp_bit_exact"]
         if "stats" in rec:
             t = time.time()
-            rec_cpu["moments_and_pvalues"] = cpu_moments_and_precision(ix, cfg, F, stats_leg.outputs)
+            rec_cpu["moments_and_pvalues"] = mp = cpu_moments_and_precision(ix, cfg, F, stats_leg.outputs)
+            parity["moments_within_1e-6"] = bool(max(mp["moments_rel_err_max"].values()) <= 1e-6)
+            parity["pvalues_within_1e-3_mlog10p"] = bool(mp["pvalue_abs_err_mlog10p_max"] <= 1e-3)
             log(f"cpu: moment / p-value checks in {time.time() - t:.1f}s: "
                 f"{rec_cpu['moments_and_pvalues']['moments_rel_err_max']}")
         rec["cpu_baseline"] = rec_cpu
@@ -610,6 +776,8 @@ def main():
             t = time.time()
             cb = cpu_batched_baseline(ix, L, batched_leg.batch, rec["batched"]["intervals_swept"], F)
             rec["batched"]["cpu_baseline"] = cb
+            parity["batched_sample_counts"] = cb["sample_counts_bit_exact"]
+            parity["batched_sample_bp"] = cb["sample_bp_bit_exact"]
             rec["batched"]["speedup_vs_cpu"] = rec["batched"]["batched_swept_per_s"] / cb["value"]
             log(f"cpu batched baseline {cb['value']:.3e} intervals swept/s on {cb['cores']} threads "
                 f"in {time.time() - t:.1f}s; sample parity {cb['sample_counts_bit_exact']}")
@@ -622,7 +790,18 @@ def main():
                 f"{time.time() - t:.1f}s")
         log(f"cpu baseline {rec_cpu['value']:.3e} intervals/s; sample parity "
             f"{rec_cpu['sample_counts_bit_exact']}")
+    if "c5" in legs:
+        ix.close()
+        rec["c5"] = c5_leg(args, parity)
+    rec["parity"] = parity
+    ok = all(parity.values())
+    if not ok:  # a wrong result publishes no rate
+        rec["status"] = "parity_failed"
+        rec["value_if_correct"] = rec["value"]
+        rec["value"] = None
     print(json.dumps(rec), flush=True)
+    if not ok:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

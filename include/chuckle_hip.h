@@ -86,6 +86,10 @@ int chk_index_shard_name(const chk_index* idx, uint32_t shard, char* buf, uint64
                          uint64_t* needed);
 int chk_index_file_name(const chk_index* idx, uint32_t fid, char* buf, uint64_t len,
                         uint64_t* needed);
+/* Attach shard and file names to an index built by chk_index_create (nshards
+ * and nfiles strings, copied), so that chk_query_open and the name queries
+ * work on it as on an opened database. */
+int chk_index_set_names(chk_index* idx, const char* const* shard_names, const char* const* file_names);
 
 /*
  * `chuckle add` (replaces add_to_database, src/tasks/build.rs:86-276): the
@@ -114,6 +118,13 @@ int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t
                      const uint32_t* start, const uint32_t* end, const uint32_t* qfile,
                      chk_query** out);
 int chk_query_free(chk_query* q);
+/* Replace a query handle's batch by a new one (arguments as chk_query_create),
+ * reusing its host staging and device buffers: a loop that uploads batch
+ * after batch allocates only when a batch outgrows the earlier ones.  Waits
+ * for work still reading the handle's buffers.  No counterpart in the
+ * reference. */
+int chk_query_reset(chk_query* q, uint32_t nfiles, uint64_t n, const uint32_t* shard,
+                    const uint32_t* start, const uint32_t* end, const uint32_t* qfile);
 /*
  * Parse BED / BED.gz query files over host threads and upload them as one
  * batch (replaces parse_file_batch, src/tasks/query/files.rs:40-111, with

@@ -54,6 +54,7 @@ def _declare(L):
     L.chk_index_info.argtypes = [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(u64), C.POINTER(u64)]
     L.chk_query_create.argtypes = [vp, u32, u64, vp, vp, vp, vp, C.POINTER(vp)]
     L.chk_query_free.argtypes = [vp]
+    L.chk_query_reset.argtypes = [vp, u32, u64, vp, vp, vp, vp]
     L.chk_query_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(u64), C.POINTER(C.c_uint32),
                                  C.POINTER(u64)]
     L.chk_sweep.argtypes = [vp, vp, vp, vp]
@@ -87,6 +88,14 @@ def _declare(L):
     L.chk_server_reserve.argtypes = [vp, u64]
     L.chk_server_free.argtypes = [vp]
     L.chk_sweep_noalloc.argtypes = [vp, u64, vp, vp, vp, vp, vp]
+    cpp = C.POINTER(C.c_char_p)
+    L.chk_index_open.argtypes = [C.c_char_p, i32, C.POINTER(vp)]
+    L.chk_index_shard_name.argtypes = [vp, u32, C.c_char_p, u64, vp]
+    L.chk_index_file_name.argtypes = [vp, u32, C.c_char_p, u64, vp]
+    L.chk_index_set_names.argtypes = [vp, cpp, cpp]
+    L.chk_query_open.argtypes = [vp, u32, cpp, vp, C.POINTER(vp)]
+    L.chk_db_add.argtypes = [C.c_char_p, C.c_char_p, u64, i32, cpp, u32, i32, i32]
+    L.chk_write_results.argtypes = [C.c_char_p, u32, cpp, u32, cpp, vp, vp, vp, vp, vp, i32]
 
 
 def _check(code):
@@ -112,8 +121,75 @@ def device_count():
     return n.value
 
 
+def _cstrs(names):
+    arr = (C.c_char_p * max(1, len(names)))()
+    for i, n in enumerate(names):
+        arr[i] = n.encode()
+    return arr
+
+
+def db_add(input_path, db_path, batch_size=0, stats=False, chroms=(), sorted_sids=False, device=0):
+    """`chuckle add`: BED files of input_path into the database at db_path."""
+    _check(lib().chk_db_add(str(input_path).encode(), str(db_path).encode(), int(batch_size),
+                            1 if stats else 0, _cstrs(list(chroms)), len(chroms),
+                            1 if sorted_sids else 0, device))
+
+
+def write_results(path, query_names, db_names, counts, bp=None, p=None, llr=None, flags=None,
+                  stats=False):
+    """StatsOutput JSON of a result matrix [len(query_names) x len(db_names)]."""
+    arrs = [np.ascontiguousarray(a) if a is not None else None for a in (counts, bp, p, llr, flags)]
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+    _check(lib().chk_write_results(str(path).encode(), len(query_names), _cstrs(query_names),
+                                   len(db_names), _cstrs(db_names), *[ptr(a) for a in arrs],
+                                   1 if stats else 0))
+
+
 class Index:
     """HBM-resident DB index (per-shard start/end streams + file ids)."""
+
+    @classmethod
+    def open(cls, db_path, device=0):
+        """Index of a database directory in the reference's format."""
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        _check(lib().chk_index_open(str(db_path).encode(), device, C.byref(h)))
+        self._h = h
+        i = self.info()
+        self.nshards, self.nfiles, self.n = i["nshards"], i["nfiles"], i["n"]
+        return self
+
+    def names(self):
+        """(shard names, file names) of an opened or named index."""
+        buf = C.create_string_buffer(4096)
+        out = []
+        for fn, k in ((lib().chk_index_shard_name, self.nshards), (lib().chk_index_file_name, self.nfiles)):
+            cur = []
+            for i in range(k):
+                _check(fn(self._h, i, buf, len(buf), None))
+                cur.append(buf.value.decode())
+            out.append(cur)
+        return tuple(out)
+
+    def set_names(self, shard_names, file_names):
+        assert len(shard_names) == self.nshards and len(file_names) == self.nfiles
+        _check(lib().chk_index_set_names(self._h, _cstrs(shard_names), _cstrs(file_names)))
+
+    def query_files(self, paths):
+        """Parse BED / BED.gz files (host threads) into one query batch;
+        returns (Query or None when every file is empty, row per path or -1)."""
+        rows = np.zeros(max(1, len(paths)), np.uint32)
+        h = C.c_void_p()
+        _check(lib().chk_query_open(self._h, len(paths), _cstrs([str(p) for p in paths]),
+                                    rows.ctypes.data_as(C.c_void_p), C.byref(h)))
+        rows = rows[:len(paths)].astype(np.int64)
+        rows[rows == 0xFFFFFFFF] = -1
+        if not h.value:
+            return None, rows
+        q = Query.__new__(Query)
+        q._h, q._index, q.nfiles = h, self, int((rows >= 0).sum())
+        q.n = None
+        return q, rows
 
     def __init__(self, shard, start, end, fid, nshards, nfiles, device=0):
         shard, start, end, fid = _u32(shard), _u32(start), _u32(end), _u32(fid)
@@ -277,6 +353,14 @@ class Query:
     @property
     def handle(self):
         return self._h
+
+    def reset(self, shard, start, end, qfile=None, nfiles=1):
+        """Replace the batch, reusing this handle's buffers."""
+        shard, start, end = _u32(shard), _u32(start), _u32(end)
+        qf = _u32(qfile) if qfile is not None else None
+        _check(lib().chk_query_reset(self._h, nfiles, len(start), _ptr(shard), _ptr(start), _ptr(end),
+                                     _ptr(qf)))
+        self.nfiles, self.n = int(nfiles), len(start)
 
     def info(self):
         """Sweep plan under the index's current tuning: files and intervals

@@ -161,6 +161,24 @@ int chk_index_file_name(const chk_index* idx, uint32_t fid, char* buf, uint64_t 
     return copy_name(idx->ix.file_names, fid, buf, len, needed, "file");
 }
 
+int chk_index_set_names(chk_index* idx, const char* const* shard_names, const char* const* file_names) {
+    return guarded([&] {
+        CHK_REQUIRE(idx && (idx->ix.nshards == 0 || shard_names) && file_names, CHK_ERR_INVALID,
+                    "NULL argument");
+        std::vector<std::string> sn, fn;
+        for (uint32_t s = 0; s < idx->ix.nshards; s++) {
+            CHK_REQUIRE(shard_names[s], CHK_ERR_INVALID, "NULL shard name");
+            sn.emplace_back(shard_names[s]);
+        }
+        for (uint32_t f = 0; f < idx->ix.nfiles; f++) {
+            CHK_REQUIRE(file_names[f], CHK_ERR_INVALID, "NULL file name");
+            fn.emplace_back(file_names[f]);
+        }
+        idx->ix.shard_names = std::move(sn);
+        idx->ix.file_names = std::move(fn);
+    });
+}
+
 int chk_db_add(const char* input, const char* db_path, uint64_t batch_size, int stats,
                const char* const* chroms, uint32_t nchroms, int sorted_sids, int device) {
     return guarded([&] {
@@ -231,6 +249,25 @@ int chk_query_create(chk_index* idx, uint32_t nfiles, uint64_t n, const uint32_t
             throw;
         }
         *out = h;
+    });
+}
+
+int chk_query_reset(chk_query* q, uint32_t nfiles, uint64_t n, const uint32_t* shard,
+                    const uint32_t* start, const uint32_t* end, const uint32_t* qfile) {
+    return guarded([&] {
+        CHK_REQUIRE(q, CHK_ERR_INVALID, "NULL query");
+        CHK_REQUIRE(nfiles > 0, CHK_ERR_INVALID, "nfiles must be positive");
+        CHK_REQUIRE(n == 0 || (shard && start && end), CHK_ERR_INVALID,
+                    "interval arrays must be non-NULL");
+        CHK_REQUIRE(nfiles == 1 || qfile || n == 0, CHK_ERR_INVALID,
+                    "qfile is required when nfiles > 1");
+        set_device(q->q.idx->device);
+        // the previous batch's copies read the staging buffers: let them finish
+        CHK_HIP(hipStreamSynchronize(q->q.stream));
+        q->q.nfiles = nfiles;
+        q->q.n_in = n;
+        chk::query_upload(q->q, shard, start, end, qfile);
+        chk::query_groups(q->q, shard, start, end, qfile);
     });
 }
 

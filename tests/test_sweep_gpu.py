@@ -293,3 +293,22 @@ def test_single_interval_query(chk, orc):
     q = tuple(np.array([v], np.uint32) for v in (0, 550, 700, 0))
     c, bp = gpu_sweep(chk, db, q, 1, 2, 1)
     assert c.tolist() == [[1, 1]] and bp.tolist() == [[50, 150]]
+
+
+def test_query_reset_reuses_handle(chk, orc):
+    # one handle, batches of changing size and file count: every reset gives
+    # the oracle's matrix
+    rng = np.random.default_rng(91)
+    S, F = 3, 40
+    db = flatten(random_files(rng, F, S, (100, 3000), 2_000_000, 20_000))
+    ix = chk.Index(*db, S, F)
+    q = None
+    for nq, per in ((3, (500, 4000)), (1, 90_000), (7, (10, 300)), (2, (20_000, 60_000))):
+        qq = flatten(random_files(rng, nq, S, per, 2_000_000, 30_000))
+        if q is None:
+            q = ix.query(qq[0], qq[1], qq[2], qq[3], nq)
+        else:
+            q.reset(qq[0], qq[1], qq[2], qq[3], nq)
+        c, bp = ix.sweep(q)
+        oc, obp = orc.sweep_ranks(db, qq, F, nq, S)
+        assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all(), nq
