@@ -212,6 +212,45 @@ def stats_leg(ix, qq, cfg, F, steps):
 
 
 C3_WIDTHS = [1, 2, 3, 4, 6, 10, 16, 25, 40, 63, 100, 158, 251, 398, 631, 1000]
+FP32_PEAK_TFLOPS = 157.3
+
+
+def c3_leg(ix, cfg, F):
+    """Precision form of the moments (C3): the whole DB's 24,000 tracks at
+    the 16 widths through the FP32 lag-limited FFT route, timed per kernel;
+    the per-width error against the exact-integer oracle comes from the CPU
+    leg."""
+    ix.profile(True)
+    regions = ("m32_coverage", "m32_fft", "m32_eval", "m32_exact")
+    for name in regions:
+        ix.profile_read(name)
+    t = time.perf_counter()
+    m, v, info = ix.moments_widths(C3_WIDTHS)
+    t_all = time.perf_counter() - t
+    k = {name: ix.profile_read(name) for name in regions}
+    ix.profile(False)
+    c3_leg.outputs = (m, v)
+    bins = info["bins"]
+    fft_ms = k["m32_fft"][1]
+    # 4096-point complex FFTs: two forward per block pair's blocks and one
+    # inverse per pair, 5 N log2 N flops each
+    blocks = 0
+    for c, (_, size) in enumerate(cfg["genome"]):
+        n = (size + 99) // 100
+        blocks += int(np.count_nonzero(~np.isnan(m[:, c, 0]))) * ((n + 3071) // 3072)
+    rows = int(np.count_nonzero(~np.isnan(m[:, :, 0])))
+    flops = blocks * 1.5 * 5 * 4096 * 12
+    alg_bytes = 4 * bins
+    return {"widths_bins": C3_WIDTHS, "tracks": rows, "bins": bins, "wall_s": t_all,
+            "kernels_ms": {n: x[1] for n, x in k.items()},
+            "fft_flops": flops, "fft_tflops": flops / (fft_ms * 1e-3) / 1e12,
+            "roofline": {"bound": "fp32", "frac_fp32": flops / (fft_ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12),
+                         "alg_bytes": alg_bytes,
+                         "frac_hbm": alg_bytes / (fft_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)},
+            "exact_widths": info["exact_widths"], "inexact_rows": info["inexact_rows"],
+            "route": "FP32 4096-point block FFTs (hop 3072) for lags < 1024, FP32 two-sum accumulation, "
+                     "FP64 variance step; (signal, width) pairs whose FP32 error bound is not small go "
+                     "through exact window sums (exact_widths)"}
 
 
 def cpu_moments_and_precision(ix, cfg, F, stats_out):
@@ -229,6 +268,7 @@ def cpu_moments_and_precision(ix, cfg, F, stats_out):
     sample = [(f, c) for c in (S - 4, S - 3) for f in range(0, F, max(1, F // 8))][:16]
     worst = {"mean_all": 0.0, "var_all": 0.0, "mean_c3": 0.0, "var_c3": 0.0}
     tracks = []
+    fp32_err = {"mean": [0.0] * len(C3_WIDTHS), "var": [0.0] * len(C3_WIDTHS)}
     order = np.lexsort((sh, fid))
     fs, ss = fid[order], sh[order]
     for f, c in sample:
@@ -248,6 +288,16 @@ def cpu_moments_and_precision(ix, cfg, F, stats_out):
         worst["mean_c3"] = max(worst["mean_c3"], rel(gm[idx], xm[idx]))
         worst["var_c3"] = max(worst["var_c3"], rel(gv[idx], xv[idx]))
         tracks.append((s[sel], e[sel], nb, ls))
+        if hasattr(c3_leg, "outputs"):  # the FP32 route vs the exact oracle at the 16 widths
+            ok = [i for i, L in enumerate(C3_WIDTHS) if L < nb]
+            xm3, xv3, _, _ = oracle.moments_exact(oracle.depth_exact(s[sel], e[sel], nb),
+                                                  np.array([C3_WIDTHS[i] for i in ok], np.uint32))
+            m3, v3 = c3_leg.outputs
+            for jj, i in enumerate(ok):
+                em = abs(m3[f, c, i] - xm3[jj]) / max(abs(xm3[jj]), 1e-300)
+                ev = abs(v3[f, c, i] - xv3[jj]) / max(abs(xv3[jj]), 1e-300)
+                fp32_err["mean"][i] = max(fp32_err["mean"][i], float(em))
+                fp32_err["var"][i] = max(fp32_err["var"][i], float(ev))
     # the reference method on the CPU (f32 depth signal, FP64 FFT moments),
     # one track per host thread as the reference's par_iter over files runs
     # them (the oracle's C calls release the GIL)
@@ -294,7 +344,11 @@ def cpu_moments_and_precision(ix, cfg, F, stats_out):
         if np.isfinite(a) and np.isfinite(b):
             dmax = max(dmax, abs(a - b))
         checked += 1
+    out_fp32 = {"widths_bins": C3_WIDTHS, "mean_rel_err_max_per_width": fp32_err["mean"],
+                "var_rel_err_max_per_width": fp32_err["var"], "tracks_checked": len(tracks)} \
+        if hasattr(c3_leg, "outputs") else None
     return {"moments_rel_err_max": worst, "moment_signals_checked": len(sample),
+            "fp32_route_vs_exact": out_fp32,
             "c3_widths_bins": C3_WIDTHS, "pvalue_cells_checked": checked,
             "pvalue_abs_err_mlog10p_max": dmax,
             "pvalues_cpu": {"value": checked / max(t_pv, 1e-9), "unit": "p-values/s", "cores": 1,
@@ -744,6 +798,8 @@ def main():
         log(f"pcie-inclusive: {rec['pcie_inclusive']['ms_per_step']:.3f} ms/step")
     if "stats" in legs:
         rec["stats"] = stats_leg(ix, qq, cfg, F, args.steps)
+        rec["stats"]["c3_fp32_route"] = c3_leg(ix, cfg, F)
+        log(f"c3 fp32 route: {rec['stats']['c3_fp32_route']['kernels_ms']}")
     if "batched" in legs and args.config == "c2":
         rec["batched"] = batched_leg(ix, cfg, F, nshards, D)
         parity["batched_rows_match_single_launches"] = rec["batched"]["rows_match_single_launches"]
@@ -768,6 +824,10 @@ def main():
             rec_cpu["moments_and_pvalues"] = mp = cpu_moments_and_precision(ix, cfg, F, stats_leg.outputs)
             parity["moments_within_1e-6"] = bool(max(mp["moments_rel_err_max"].values()) <= 1e-6)
             parity["pvalues_within_1e-3_mlog10p"] = bool(mp["pvalue_abs_err_mlog10p_max"] <= 1e-3)
+            if mp["fp32_route_vs_exact"]:
+                e3 = mp["fp32_route_vs_exact"]
+                parity["c3_fp32_route_within_1e-6"] = bool(
+                    max(e3["mean_rel_err_max_per_width"] + e3["var_rel_err_max_per_width"]) <= 1e-6)
             log(f"cpu: moment / p-value checks in {time.time() - t:.1f}s: "
                 f"{rec_cpu['moments_and_pvalues']['moments_rel_err_max']}")
         rec["cpu_baseline"] = rec_cpu

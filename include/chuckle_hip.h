@@ -190,6 +190,28 @@ int chk_moments_upload(chk_index* idx, const int64_t* slot_off, const uint32_t* 
  */
 int chk_moments_build(chk_index* idx, int32_t* rows, int32_t* exact_rows);
 
+/*
+ * Precision form of the moments (no counterpart in the reference, whose
+ * tables come from one FP64 FFT per chromosome, src/fourier/moments.rs:421-466):
+ * mean and variance of every (DB file, shard) at nwidths block sizes of 1..1024
+ * bins, from FP32 FFTs of 4096-point blocks (hop 3072) of the centred
+ * coverage that give the autocorrelation lags below 1024, accumulated in FP32
+ * two-sum (compensated) arithmetic; FP64 only in the variance step
+ * E[x^2] - E[x]^2.  Host outputs mean / var of nfiles x nshards x nwidths
+ * (NaN where the file has no intervals on the shard, the shard is outside the
+ * genome table, or L >= n_bins).  inexact_rows (optional) receives the number
+ * of signals with a centred coverage value of 2^24 or more (not exact in
+ * FP32), bins (optional) the bins processed, lags (optional, nfiles x
+ * nshards x 1024) the lags A[0..1023] as FP64.  Signals of at most 8192 bins,
+ * and widths whose FP32 error bound (2.5e-7 A[0] L^1.5) exceeds 5e-7 of the
+ * signal's variance sum, are evaluated by exact integer window sums instead;
+ * exact_widths (optional) receives how many (signal, width) pairs were.
+ * Needs the genome table.
+ */
+int chk_moments_widths(chk_index* idx, uint32_t nwidths, const uint32_t* widths, double* mean,
+                       double* var, uint32_t* inexact_rows, uint64_t* bins, double* lags,
+                       uint64_t* exact_widths);
+
 /* Copy the installed tables to the host: *total receives the entry count;
  * arrays may be NULL (slot arrays hold nfiles*nshards entries). */
 int chk_moments_download(chk_index* idx, uint64_t* total, int64_t* slot_off, uint32_t* slot_len,

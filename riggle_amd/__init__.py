@@ -55,6 +55,8 @@ def _declare(L):
     L.chk_query_create.argtypes = [vp, u32, u64, vp, vp, vp, vp, C.POINTER(vp)]
     L.chk_query_free.argtypes = [vp]
     L.chk_query_reset.argtypes = [vp, u32, u64, vp, vp, vp, vp]
+    L.chk_moments_widths.argtypes = [vp, u32, vp, vp, vp, C.POINTER(u32), C.POINTER(u64), vp,
+                                     C.POINTER(u64)]
     L.chk_query_info.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(u64), C.POINTER(C.c_uint32),
                                  C.POINTER(u64)]
     L.chk_sweep.argtypes = [vp, vp, vp, vp]
@@ -266,6 +268,23 @@ class Index:
         rows, ex = C.c_int32(), C.c_int32()
         _check(lib().chk_moments_build(self._h, C.byref(rows), C.byref(ex)))
         return rows.value, ex.value
+
+    def moments_widths(self, widths, lags=False):
+        """FP32 lag-limited route: (mean, var) [files x shards x widths] at the
+        given block sizes (<= 1024 bins), plus {"inexact_rows", "bins"} and,
+        with lags=True, the autocorrelation lags [files x shards x 1024]."""
+        w = np.ascontiguousarray(widths, np.uint32)
+        shp = (self.nfiles, self.nshards, len(w))
+        m = np.zeros(shp, np.float64)
+        v = np.zeros(shp, np.float64)
+        lg = np.zeros((self.nfiles, self.nshards, 1024), np.float64) if lags else None
+        bad, bins, ex = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(lib().chk_moments_widths(self._h, len(w), w.ctypes.data_as(C.c_void_p),
+                                        m.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p),
+                                        C.byref(bad), C.byref(bins),
+                                        lg.ctypes.data_as(C.c_void_p) if lags else None, C.byref(ex)))
+        info = {"inexact_rows": bad.value, "bins": bins.value, "exact_widths": ex.value}
+        return (m, v, info, lg) if lags else (m, v, info)
 
     def download_moments(self):
         """(slot_off [F*S], slot_len [F*S], mean, var) of the installed tables."""

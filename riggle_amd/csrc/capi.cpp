@@ -403,6 +403,16 @@ int chk_moments_build(chk_index* idx, int32_t* rows, int32_t* exact_rows) {
     });
 }
 
+int chk_moments_widths(chk_index* idx, uint32_t nwidths, const uint32_t* widths, double* mean,
+                       double* var, uint32_t* inexact_rows, uint64_t* bins, double* lags,
+                       uint64_t* exact_widths) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");
+        set_device(idx->ix.device);
+        chk::moments_widths(idx->ix, widths, nwidths, mean, var, inexact_rows, bins, lags, exact_widths);
+    });
+}
+
 int chk_moments_download(chk_index* idx, uint64_t* total, int64_t* slot_off, uint32_t* slot_len,
                          double* mean, double* var) {
     return guarded([&] {

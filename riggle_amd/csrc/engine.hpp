@@ -254,6 +254,14 @@ void moments_upload(Index& ix, const int64_t* slot_off, const uint32_t* slot_len
 // Compact moment tables of every (file, shard) from the index's intervals.
 void moments_build(Index& ix);
 void moments_download(Index& ix, int64_t* slot_off, uint32_t* slot_len, double* mean, double* var);
+// FP32 lag-limited route: mean / var of every (file, shard) at block sizes
+// widths[0..nw) <= 1024 bins, [F x S x nw] host arrays (NaN where a slot has
+// no intervals or L >= n_bins); lags_out (optional, [F x S x 1024]) receives
+// the FP64 values of the autocorrelation lags; exact_widths the (signal,
+// width) pairs evaluated by exact window sums (short signals, or an FP32
+// error bound above 5e-7 of the variance sum).
+void moments_widths(Index& ix, const uint32_t* widths, uint32_t nw, double* mean, double* var,
+                    uint32_t* inexact_rows, uint64_t* bins, double* lags_out, uint64_t* exact_widths);
 // p-values for every cell with count > 0 (device arrays of nfiles(q) x F).
 void pvalues_run(Query& q, const uint64_t* d_counts, const uint64_t* d_bp, double* d_p,
                  double* d_llr, uint8_t* d_flags, hipStream_t st);

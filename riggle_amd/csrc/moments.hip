@@ -1332,4 +1332,514 @@ void moments_download(Index& ix, int64_t* slot_off, uint32_t* slot_len, double* 
         CHK_HIP(hipMemcpy(var, M.var.p, M.total * 8, hipMemcpyDeviceToHost));
 }
 
+// ------------------------------------------------------------ FP32 lag-limited route
+//
+// The C3 precision form: moments at a few block sizes L <= LB_LAGS (the
+// configurations' 16 widths of 1..1000 bins) need only the lags m < L of the
+// autocorrelation, so A[m] for m < LB_LAGS comes from blocked FP32 FFTs
+// instead of one FP64 transform of the whole chromosome:
+//   A[m] = sum_b sum_{t < LB_HOP} H[b HOP + t] H[b HOP + t + m],
+// each block b one complex FFT of LB_N = 4096 points of z = x + i y with
+// x = H[b HOP .. +HOP) (zero beyond HOP) and y = H[b HOP .. +N), since
+// c[m] = sum_t x[t] y[t + m] never wraps for m < N - HOP.  Two blocks share
+// each inverse FFT (conj(X1) Y1 + i conj(X2) Y2 gives c1 + i c2).  The FFTs
+// are three Stockham radix-16 passes through LDS (16 points per thread,
+// natural order in and out); a block's lags are added into FP32 two-sum
+// (compensated) accumulators, the work group's partial sums are combined in
+// the same two-sum arithmetic in a fixed order, and only the variance step
+// runs in FP64:
+//   S2_full(L) = L A[0] + 2 sum_{1<=d<L} (L - d) A[d]   (zero-padded H)
+//   S2_H(L)    = S2_full - sum_{1<=m<L} PH[m]^2 - sum_{1<=k<L} SUF[k]^2
+//   S1_H(L)    = sum_{j=n-L+1..n} PH[j] - sum_{j<L} PH[j]
+// with PH the prefix sums of H and SUF[k] the sum of its last k values, both
+// exact integers over the first and last L bins; mean = (S1_H / n_w + c L) /
+// 100, var = (S2_H / n_w - (S1_H / n_w)^2) / 1e4, n_w = n - L + 1.
+namespace {
+
+constexpr uint32_t LB_N = 4096, LB_HOP = 3072, LB_LAGS = LB_N - LB_HOP;
+constexpr uint32_t LB_PAIRS = 8;  // block pairs per work group (49,152 bins)
+
+__device__ inline uint32_t lpad(uint32_t i) { return i + (i >> 4); }
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// forward 4-point DFT in place
+__device__ inline void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+    const float2 s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = csub(a1, a3);
+    a0 = cadd(s02, s13);
+    a2 = csub(s02, s13);
+    a1 = make_float2(d02.x + d13.y, d02.y - d13.x);  // d02 - i d13
+    a3 = make_float2(d02.x - d13.y, d02.y + d13.x);  // d02 + i d13
+}
+
+// forward 16-point DFT: v (natural order) -> w (natural order), as 4 x 4
+// with the internal twiddles exp(-2 pi i n2 k1 / 16)
+__device__ inline void dft16(float2 (&v)[16], float2 (&w)[16]) {
+    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, R2 = 0.70710678118654752f;
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // v[n2 + 4 k1] *= W16^(n2 k1)
+    v[5] = cmul(v[5], make_float2(C1, -S1));    // n2 1, k1 1
+    v[9] = cmul(v[9], make_float2(R2, -R2));    // n2 1, k1 2
+    v[13] = cmul(v[13], make_float2(S1, -C1));  // n2 1, k1 3
+    v[6] = cmul(v[6], make_float2(R2, -R2));    // n2 2, k1 1
+    v[10] = make_float2(v[10].y, -v[10].x);     // n2 2, k1 2: -i
+    v[14] = cmul(v[14], make_float2(-R2, -R2)); // n2 2, k1 3
+    v[7] = cmul(v[7], make_float2(S1, -C1));    // n2 3, k1 1
+    v[11] = cmul(v[11], make_float2(-R2, -R2)); // n2 3, k1 2
+    v[15] = cmul(v[15], make_float2(-C1, S1));  // n2 3, k1 3: W16^9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        float2 a0 = v[4 * k1], a1 = v[4 * k1 + 1], a2 = v[4 * k1 + 2], a3 = v[4 * k1 + 3];
+        dft4(a0, a1, a2, a3);
+        w[k1] = a0;
+        w[k1 + 4] = a1;
+        w[k1 + 8] = a2;
+        w[k1 + 12] = a3;
+    }
+}
+
+// Forward 4096-point FFT: on entry thread j holds v[k] = z[j + 256 k]; on
+// exit it holds w[k] = Z[j + 256 k] and, when `publish`, the LDS holds Z in
+// natural order (padded indices).  T[m] = exp(-2 pi i m / 4096) in FP32.
+__device__ inline void fft4096(float2 (&v)[16], float2 (&w)[16], float2* lds, const float2* T,
+                               bool publish) {
+    const uint32_t j = threadIdx.x;
+    // pass 0: stride 1, no twiddles
+    dft16(v, w);
+#pragma unroll
+    for (int k = 0; k < 16; k++) lds[lpad(j * 16 + k)] = w[k];
+    __syncthreads();
+    // pass 1: stride 16
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = lds[lpad(j + 256 * k)];
+    __syncthreads();
+#pragma unroll
+    for (int k = 1; k < 16; k++) v[k] = cmul(v[k], T[((j & 15) * k * 16) & (LB_N - 1)]);
+    dft16(v, w);
+    {
+        const uint32_t base = (j >> 4) * 256 + (j & 15);
+#pragma unroll
+        for (int k = 0; k < 16; k++) lds[lpad(base + 16 * k)] = w[k];
+    }
+    __syncthreads();
+    // pass 2: stride 256; output index j + 256 k (natural order)
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = lds[lpad(j + 256 * k)];
+    __syncthreads();
+#pragma unroll
+    for (int k = 1; k < 16; k++) v[k] = cmul(v[k], T[(j * k) & (LB_N - 1)]);
+    dft16(v, w);
+    if (publish) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) lds[lpad(j + 256 * k)] = w[k];
+        __syncthreads();
+    }
+}
+
+// FP32 two-sum: s + v = t + e exactly; the error goes into c
+__device__ inline void two_sum_add(float& s, float& c, float v) {
+    const float t = s + v;
+    const float bp = t - s;
+    const float e = (s - (t - bp)) + (v - bp);
+    s = t;
+    c += e;
+}
+
+struct LagArgs {
+    const int32_t* G;       // coverage rows, row stride ldg
+    uint32_t ldg, n;
+    const int64_t* cen;     // centring constant per row
+    const float2* T;        // exp(-2 pi i m / 4096), m < 4096
+    uint32_t npairs, nchunk;
+    float* part;            // [row][chunk][LB_LAGS][2]: two-sum (sum, compensation)
+    uint32_t* inexact;      // per row: set when some |H| >= 2^24 (not exact in FP32)
+};
+
+// The spectrum of block b's autocorrelation product conj(X) Y for the
+// thread's 16 frequencies, from Z = FFT(x + i y) held in w (own) and LDS
+// (partners N - k).
+__device__ inline void lag_spectrum(const float2 (&w)[16], const float2* lds, float2 (&P)[16]) {
+    const uint32_t j = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const uint32_t k = j + 256 * q;
+        const float2 a = w[q];
+        const float2 bb = cconj(lds[lpad((LB_N - k) & (LB_N - 1))]);
+        const float2 sx = cadd(a, bb), dy = csub(a, bb);
+        // conj(X) Y = conj(a + bb) (a - bb) (-i) / 4
+        const float2 pr = cmul(cconj(sx), dy);
+        P[q] = make_float2(0.25f * pr.y, -0.25f * pr.x);
+    }
+}
+
+__device__ inline void lag_load(const LagArgs& a, const int32_t* row, int64_t c, uint32_t blk,
+                                float2 (&v)[16], uint32_t& bad) {
+    const uint32_t j = threadIdx.x;
+    const uint64_t t0 = (uint64_t)blk * LB_HOP;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint32_t t = j + 256 * k;
+        const uint64_t gi = t0 + t;
+        float h = 0.f;
+        if (gi < a.n) {
+            const int64_t hv = (int64_t)row[gi] - c;
+            bad |= (hv >= (1 << 24) || hv <= -(1 << 24)) ? 1u : 0u;
+            h = (float)hv;
+        }
+        v[k] = make_float2(t < LB_HOP ? h : 0.f, h);
+    }
+}
+
+__global__ __launch_bounds__(NT) void lag_fft32(LagArgs a) {
+    __shared__ float2 lds[LB_N + LB_N / 16];
+    const uint32_t r = blockIdx.y, chunk = blockIdx.x, j = threadIdx.x;
+    const int32_t* row = a.G + (size_t)r * a.ldg;
+    const int64_t c = a.cen[r];
+    const uint32_t nblk = (a.n + LB_HOP - 1) / LB_HOP;
+    float s[4] = {0.f, 0.f, 0.f, 0.f}, cc[4] = {0.f, 0.f, 0.f, 0.f};
+    uint32_t bad = 0;
+    const uint32_t p0 = chunk * LB_PAIRS, p1 = min(a.npairs, p0 + LB_PAIRS);
+    for (uint32_t p = p0; p < p1; p++) {
+        float2 v[16], w[16], P1[16], P2[16];
+        lag_load(a, row, c, 2 * p, v, bad);
+        fft4096(v, w, lds, a.T, true);
+        lag_spectrum(w, lds, P1);
+        __syncthreads();  // the partner reads are done before the LDS is reused
+        if (2 * p + 1 < nblk) {
+            lag_load(a, row, c, 2 * p + 1, v, bad);
+            fft4096(v, w, lds, a.T, true);
+            lag_spectrum(w, lds, P2);
+            __syncthreads();
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; q++) P2[q] = make_float2(0.f, 0.f);
+        }
+        // inverse of P1 + i P2 as conj(FFT(conj(.))): c1 + i c2, times 4096
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = make_float2(P1[q].x - P2[q].y, -(P1[q].y + P2[q].x));
+        fft4096(v, w, lds, a.T, false);
+        // lags m = j + 256 q, q < 4: c1[m] = Re / N, c2[m] = Im / N (conj flips Im)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            two_sum_add(s[q], cc[q], w[q].x * (1.0f / LB_N));
+            two_sum_add(s[q], cc[q], -w[q].y * (1.0f / LB_N));
+        }
+    }
+    float* out = a.part + ((size_t)r * a.nchunk + chunk) * LB_LAGS * 2;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        out[2 * (j + 256 * q)] = s[q];
+        out[2 * (j + 256 * q) + 1] = cc[q];
+    }
+    if (bad) atomicOr(&a.inexact[r], 1u);
+}
+
+struct Lag32Eval {
+    const int32_t* G;
+    uint32_t ldg, n;
+    const int64_t* cen;
+    const int64_t* sumg;
+    const float* part;
+    uint32_t nchunk;
+    const uint32_t* widths;
+    uint32_t nw;
+    const int64_t* out_slot;  // per row: first output entry
+    double* mean;
+    double* var;
+    double* A_out;            // optional [row][LB_LAGS]: the FP64 lags (tests), or null
+    uint32_t* redo;           // [row][nw]: 1 when the width is evaluated exactly instead
+};
+
+// Rows this short (or widths whose FP32 error bound is not small against
+// S2_H) are evaluated by exact window sums instead (lag_exact).
+constexpr uint32_t LB_EXACT_N = 8192;
+// FP32 FFT error of the weighted lag sum: relative to A[0] L^1.5 it stays
+// below 1.7e-7 on white-noise tracks (measured); 2.5e-7 with margin.
+constexpr double LB_ERR = 2.5e-7;
+constexpr double LB_TOL = 5.0e-7;
+
+// One block per row: lags from the work groups' two-sum partials (fixed
+// order), exact edge prefix sums, then mean / variance per width.
+__global__ __launch_bounds__(NT) void lag_eval32(Lag32Eval a) {
+    __shared__ double A[LB_LAGS];
+    __shared__ int64_t PHl[LB_LAGS + 1], SUF[LB_LAGS + 1];
+    const uint32_t r = blockIdx.x, j = threadIdx.x, n = a.n;
+    const float* pr = a.part + (size_t)r * a.nchunk * LB_LAGS * 2;
+    for (uint32_t m = j; m < LB_LAGS; m += NT) {
+        float S = 0.f, Cp = 0.f;
+        for (uint32_t k = 0; k < a.nchunk; k++) {
+            two_sum_add(S, Cp, pr[((size_t)k * LB_LAGS + m) * 2]);
+            Cp += pr[((size_t)k * LB_LAGS + m) * 2 + 1];
+        }
+        A[m] = (double)S + (double)Cp;
+        if (a.A_out) a.A_out[(size_t)r * LB_LAGS + m] = A[m];
+    }
+    const int32_t* row = a.G + (size_t)r * a.ldg;
+    const int64_t c = a.cen[r];
+    const uint32_t E = min(LB_LAGS, n);  // edge bins needed (L <= n - 1 < E + 1)
+    if (j == 0) {
+        int64_t acc = 0;
+        PHl[0] = 0;
+        for (uint32_t m = 1; m <= E; m++) {
+            acc += (int64_t)row[m - 1] - c;
+            PHl[m] = acc;
+        }
+        acc = 0;
+        SUF[0] = 0;
+        for (uint32_t k = 1; k <= E; k++) {
+            acc += (int64_t)row[n - k] - c;
+            SUF[k] = acc;
+        }
+    }
+    __syncthreads();
+    const int64_t PHn = a.sumg[r] - c * (int64_t)n;
+    for (uint32_t wi = j; wi < a.nw; wi += NT) {
+        const uint32_t L = a.widths[wi];
+        const int64_t o = a.out_slot[r] + wi;
+        if (L == 0 || L >= n || L > LB_LAGS) {
+            a.mean[o] = NAN;
+            a.var[o] = NAN;
+            continue;
+        }
+        // S1_H = L PH[n] - sum_{k<L} SUF[k] - sum_{m<L} PH[m]
+        int64_t s1 = (int64_t)L * PHn;
+        __int128 def = 0;
+        for (uint32_t k = 0; k < L; k++) {
+            s1 -= SUF[k] + PHl[k];
+            def += (__int128)PHl[k] * PHl[k] + (__int128)SUF[k] * SUF[k];
+        }
+        double s2 = (double)L * A[0];
+        for (uint32_t d = 1; d < L; d++) s2 += 2.0 * (double)(L - d) * A[d];
+        s2 -= (double)def;
+        const double nwin = (double)(n - L + 1);
+        const double m1 = (double)s1 / nwin;
+        const double v = s2 / nwin - m1 * m1;
+        a.mean[o] = (m1 + (double)c * (double)L) / 100.0;
+        a.var[o] = (v > 0.0 ? v : 0.0) / 1.0e4;
+        // error bound of the FP32 lags in S2_H, against the variance sum it feeds
+        const double err = LB_ERR * A[0] * (double)L * sqrt((double)L);
+        const double vs = (v > 0.0 ? v : 0.0) * nwin;
+        a.redo[(size_t)r * a.nw + wi] = (n <= LB_EXACT_N || !(err <= LB_TOL * vs)) ? 1u : 0u;
+    }
+}
+
+struct LagExact {
+    const int32_t* G;
+    uint32_t ldg, n;
+    const int64_t* cen;
+    const uint32_t* widths;
+    uint32_t nw;
+    const uint32_t* redo;     // [row][nw]
+    const uint32_t* rows;     // rows with at least one width to redo
+    const int64_t* out_slot;
+    double* mean;
+    double* var;
+};
+
+// Exact S1_H, S2_H by window sums for the widths flagged in `redo`: thread t
+// walks the windows k of its run [k0, k1), w_{k+1} = w_k + H[k+L] - H[k],
+// from w_{k0} summed directly (L <= 1024 values); S2 in 128-bit integers.
+__global__ __launch_bounds__(NT) void lag_exact(LagExact a) {
+    __shared__ __int128 l2[NT];
+    __shared__ int64_t l1[NT];
+    const uint32_t r = a.rows[blockIdx.x], j = threadIdx.x, n = a.n;
+    const int32_t* row = a.G + (size_t)r * a.ldg;
+    const int64_t c = a.cen[r];
+    const uint32_t run = (n + NT - 1) / NT;
+    const uint32_t b0 = min(n, j * run), b1 = min(n, b0 + run);
+    for (uint32_t wi = 0; wi < a.nw; wi++) {
+        if (!a.redo[(size_t)r * a.nw + wi]) continue;
+        const uint32_t L = a.widths[wi];
+        const int64_t o = a.out_slot[r] + wi;
+        if (L == 0 || L >= n) continue;  // NaN already written
+        const uint32_t K = n - L + 1;    // windows k = 0..n-L
+        const uint32_t k0 = min(K, b0), k1 = min(K, b1);
+        int64_t s1 = 0;
+        __int128 s2 = 0;
+        if (k0 < k1) {
+            int64_t w = 0;
+            for (uint32_t t = k0; t < k0 + L; t++) w += (int64_t)row[t] - c;
+            for (uint32_t k = k0; k < k1; k++) {
+                s1 += w;
+                s2 += (__int128)w * w;
+                if (k + 1 < k1) w += ((int64_t)row[k + L] - c) - ((int64_t)row[k] - c);
+            }
+        }
+        l1[j] = s1;
+        l2[j] = s2;
+        __syncthreads();
+        if (j == 0) {
+            int64_t S1 = 0;
+            __int128 S2 = 0;
+            for (uint32_t t = 0; t < NT; t++) {
+                S1 += l1[t];
+                S2 += l2[t];
+            }
+            const double nwin = (double)K;
+            const double m1 = (double)S1 / nwin;
+            const double v = (double)S2 / nwin - m1 * m1;
+            a.mean[o] = (m1 + (double)c * (double)L) / 100.0;
+            a.var[o] = (v > 0.0 ? v : 0.0) / 1.0e4;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+void moments_widths(Index& ix, const uint32_t* widths, uint32_t nw, double* mean, double* var,
+                    uint32_t* inexact_rows, uint64_t* bins_out, double* lags_out, uint64_t* exact_widths) {
+    Moments& M = ix.mom;
+    CHK_REQUIRE(M.genome, CHK_ERR_INVALID, "set the genome table before computing moments");
+    CHK_REQUIRE(nw > 0 && widths && mean && var, CHK_ERR_INVALID, "widths and outputs are required");
+    for (uint32_t w = 0; w < nw; w++)
+        CHK_REQUIRE(widths[w] >= 1 && widths[w] <= LB_LAGS, CHK_ERR_INVALID,
+                    "widths of the FP32 route are 1.." + std::to_string(LB_LAGS) + " bins");
+    const uint32_t S = ix.nshards, F = ix.nfiles;
+    hipStream_t st = ix.stream;
+    const size_t nout = (size_t)F * S * nw;
+    std::vector<float2> th(LB_N);
+    for (uint32_t m = 0; m < LB_N; m++) {
+        const double ang = -2.0 * M_PI * (double)m / (double)LB_N;
+        th[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    DevBuf<float2> T;
+    DevBuf<uint32_t> d_w, d_bad;
+    DevBuf<double> d_mean, d_var, d_lags;
+    DevBuf<int32_t> slot_of_fid, D;
+    DevBuf<int64_t> cen, sumg, gseg, out_slot;
+    DevBuf<dd> sumh2;
+    DevBuf<float> part;
+    DevBuf<uint32_t> redo, redo_rows;
+    uint64_t nexact_w = 0;
+    T.reserve(LB_N);
+    d_w.reserve(nw);
+    d_mean.reserve(nout);
+    d_var.reserve(nout);
+    if (lags_out) d_lags.reserve((size_t)F * S * LB_LAGS);
+    slot_of_fid.reserve(F);
+    CHK_HIP(hipMemcpyAsync(T.p, th.data(), LB_N * sizeof(float2), hipMemcpyHostToDevice, st));
+    CHK_HIP(hipMemcpyAsync(d_w.p, widths, nw * 4, hipMemcpyHostToDevice, st));
+    // slots without a table stay NaN
+    std::vector<double> nanv(nout, NAN);
+    CHK_HIP(hipMemcpyAsync(d_mean.p, nanv.data(), nout * 8, hipMemcpyHostToDevice, st));
+    CHK_HIP(hipMemcpyAsync(d_var.p, nanv.data(), nout * 8, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> sof(F, -1);
+    uint32_t nbad_rows = 0;
+    uint64_t bins = 0;
+    for (uint32_t s = 0; s < S; s++) {
+        const uint32_t n = M.nbins[s];
+        if (n < 2) continue;
+        std::vector<uint32_t> files;
+        for (uint32_t f = 0; f < F; f++)
+            if (ix.fs_count[(size_t)f * S + s]) files.push_back(f);
+        if (files.empty()) continue;
+        const uint32_t ldg = n + 1;
+        const uint32_t nblk = (n + LB_HOP - 1) / LB_HOP, npairs = (nblk + 1) / 2;
+        const uint32_t nchunk = (npairs + LB_PAIRS - 1) / LB_PAIRS;
+        size_t free_b = 0, total_b = 0;
+        CHK_HIP(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t row_bytes = (uint64_t)ldg * 4 + (uint64_t)nchunk * LB_LAGS * 8 + 64;
+        const uint32_t B = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>({files.size(), (uint64_t)(free_b / 4) / row_bytes, 2048}));
+        D.reserve((size_t)B * ldg);
+        part.reserve((size_t)B * nchunk * LB_LAGS * 2);
+        cen.reserve(B);
+        sumg.reserve(B);
+        sumh2.reserve(B);
+        gseg.reserve(B);
+        out_slot.reserve(B);
+        d_bad.reserve(B);
+        redo.reserve((size_t)B * nw);
+        const uint32_t seg = (ldg + CH - 1) / CH * CH;  // one segment: no evaluation split here
+        const size_t NSUB = (size_t)ix.ngroups * ix.nlayers;
+        const LayerSeg& Ll = ix.lseg[(size_t)s * NSUB + NSUB - 1];
+        const uint64_t soff = ix.lseg[(size_t)s * NSUB].off, scnt = Ll.off + Ll.n - soff;
+        for (size_t f0 = 0; f0 < files.size(); f0 += B) {
+            const uint32_t nb = (uint32_t)std::min<size_t>(B, files.size() - f0);
+            std::vector<int64_t> oo(nb);
+            for (uint32_t k = 0; k < nb; k++) {
+                sof[files[f0 + k]] = (int32_t)k;
+                oo[k] = ((int64_t)files[f0 + k] * S + s) * nw;
+            }
+            CHK_HIP(hipMemcpyAsync(slot_of_fid.p, sof.data(), F * 4, hipMemcpyHostToDevice, st));
+            CHK_HIP(hipMemcpyAsync(out_slot.p, oo.data(), nb * 8, hipMemcpyHostToDevice, st));
+            CHK_HIP(hipMemsetAsync(D.p, 0, (size_t)nb * ldg * 4, st));
+            CHK_HIP(hipMemsetAsync(d_bad.p, 0, (size_t)nb * 4, st));
+            hipEvent_t t0 = ix.prof.begin("m32_coverage", st);
+            const unsigned grid = (unsigned)std::min<uint64_t>((scnt + NT - 1) / NT, (uint64_t)ix.ncu * 16);
+            if (grid)
+                hipLaunchKernelGGL(impulses, dim3(grid), dim3(NT), 0, st, ix.l_start.p + soff,
+                                   ix.l_end.p + soff, ix.l_fid.p + soff, scnt, slot_of_fid.p, n, ldg, D.p);
+            hipLaunchKernelGGL(coverage_rows, dim3(nb), dim3(NT), 0, st, D.p, ldg, n, cen.p, sumg.p,
+                               sumh2.p, seg, 1u, gseg.p);
+            CHK_HIP(hipGetLastError());
+            ix.prof.end("m32_coverage", t0, st);
+            LagArgs la{D.p, ldg, n, cen.p, T.p, npairs, nchunk, part.p, d_bad.p};
+            t0 = ix.prof.begin("m32_fft", st);
+            hipLaunchKernelGGL(lag_fft32, dim3(nchunk, nb), dim3(NT), 0, st, la);
+            CHK_HIP(hipGetLastError());
+            ix.prof.end("m32_fft", t0, st);
+            Lag32Eval ea{D.p, ldg, n, cen.p, sumg.p, part.p, nchunk, d_w.p, nw, out_slot.p,
+                         d_mean.p, d_var.p, nullptr, redo.p};
+            DevBuf<double> lrow;
+            if (lags_out) {
+                lrow.reserve((size_t)nb * LB_LAGS);
+                ea.A_out = lrow.p;
+            }
+            t0 = ix.prof.begin("m32_eval", st);
+            hipLaunchKernelGGL(lag_eval32, dim3(nb), dim3(NT), 0, st, ea);
+            CHK_HIP(hipGetLastError());
+            ix.prof.end("m32_eval", t0, st);
+            // widths whose FP32 error bound is not small (and short rows):
+            // exact window sums
+            std::vector<uint32_t> rf((size_t)nb * nw), rl;
+            CHK_HIP(hipMemcpyAsync(rf.data(), redo.p, rf.size() * 4, hipMemcpyDeviceToHost, st));
+            CHK_HIP(hipStreamSynchronize(st));
+            for (uint32_t k = 0; k < nb; k++) {
+                bool any = false;
+                for (uint32_t w = 0; w < nw; w++) {
+                    any |= rf[(size_t)k * nw + w] != 0;
+                    nexact_w += rf[(size_t)k * nw + w] != 0;
+                }
+                if (any) rl.push_back(k);
+            }
+            if (!rl.empty()) {
+                redo_rows.reserve(rl.size());
+                CHK_HIP(hipMemcpyAsync(redo_rows.p, rl.data(), rl.size() * 4, hipMemcpyHostToDevice, st));
+                LagExact xa{D.p, ldg, n, cen.p, d_w.p, nw, redo.p, redo_rows.p, out_slot.p, d_mean.p, d_var.p};
+                t0 = ix.prof.begin("m32_exact", st);
+                hipLaunchKernelGGL(lag_exact, dim3((uint32_t)rl.size()), dim3(NT), 0, st, xa);
+                CHK_HIP(hipGetLastError());
+                ix.prof.end("m32_exact", t0, st);
+            }
+            std::vector<uint32_t> bad(nb);
+            CHK_HIP(hipMemcpyAsync(bad.data(), d_bad.p, nb * 4, hipMemcpyDeviceToHost, st));
+            if (lags_out) {
+                for (uint32_t k = 0; k < nb; k++)
+                    CHK_HIP(hipMemcpyAsync(lags_out + ((size_t)files[f0 + k] * S + s) * LB_LAGS,
+                                           lrow.p + (size_t)k * LB_LAGS, LB_LAGS * 8,
+                                           hipMemcpyDeviceToHost, st));
+            }
+            CHK_HIP(hipStreamSynchronize(st));
+            for (uint32_t k = 0; k < nb; k++) {
+                nbad_rows += bad[k] ? 1 : 0;
+                sof[files[f0 + k]] = -1;
+            }
+            bins += (uint64_t)nb * n;
+        }
+    }
+    CHK_HIP(hipMemcpyAsync(mean, d_mean.p, nout * 8, hipMemcpyDeviceToHost, st));
+    CHK_HIP(hipMemcpyAsync(var, d_var.p, nout * 8, hipMemcpyDeviceToHost, st));
+    CHK_HIP(hipStreamSynchronize(st));
+    if (inexact_rows) *inexact_rows = nbad_rows;
+    if (bins_out) *bins_out = bins;
+    if (exact_widths) *exact_widths = nexact_w;
+}
+
 }  // namespace chk
