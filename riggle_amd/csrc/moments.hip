@@ -1359,7 +1359,6 @@ namespace {
 constexpr uint32_t LB_N = 4096, LB_HOP = 3072, LB_LAGS = LB_N - LB_HOP;
 constexpr uint32_t LB_PAIRS = 8;  // block pairs per work group (49,152 bins)
 
-__device__ inline uint32_t lpad(uint32_t i) { return i + (i >> 4); }
 __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ inline float2 cmul(float2 a, float2 b) {
@@ -1376,69 +1375,65 @@ __device__ inline void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
     a3 = make_float2(d02.x - d13.y, d02.y + d13.x);  // d02 + i d13
 }
 
-// forward 16-point DFT: v (natural order) -> w (natural order), as 4 x 4
-// with the internal twiddles exp(-2 pi i n2 k1 / 16)
-__device__ inline void dft16(float2 (&v)[16], float2 (&w)[16]) {
-    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, R2 = 0.70710678118654752f;
+// Forward 8-point DFT in place, as 2 x 4 (n = 4 n1 + n2) with the internal
+// twiddles exp(-2 pi i n2 k1 / 8): input v[n] in natural order, output
+// V[k1 + 2 k2] in register 4 k1 + k2 (digit-transposed; the callers' store
+// addresses undo it, so no register moves are needed).
+__device__ inline void dft8(float2 (&v)[8]) {
+    constexpr float R2 = 0.70710678118654752f;
 #pragma unroll
-    for (int n2 = 0; n2 < 4; n2++) dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
-    // v[n2 + 4 k1] *= W16^(n2 k1)
-    v[5] = cmul(v[5], make_float2(C1, -S1));    // n2 1, k1 1
-    v[9] = cmul(v[9], make_float2(R2, -R2));    // n2 1, k1 2
-    v[13] = cmul(v[13], make_float2(S1, -C1));  // n2 1, k1 3
-    v[6] = cmul(v[6], make_float2(R2, -R2));    // n2 2, k1 1
-    v[10] = make_float2(v[10].y, -v[10].x);     // n2 2, k1 2: -i
-    v[14] = cmul(v[14], make_float2(-R2, -R2)); // n2 2, k1 3
-    v[7] = cmul(v[7], make_float2(S1, -C1));    // n2 3, k1 1
-    v[11] = cmul(v[11], make_float2(-R2, -R2)); // n2 3, k1 2
-    v[15] = cmul(v[15], make_float2(-C1, S1));  // n2 3, k1 3: W16^9
+    for (int n2 = 0; n2 < 4; n2++) {
+        const float2 x = v[n2], y = v[4 + n2];
+        v[n2] = cadd(x, y);
+        v[4 + n2] = csub(x, y);
+    }
+    v[5] = cmul(v[5], make_float2(R2, -R2));   // W8^1
+    v[6] = make_float2(v[6].y, -v[6].x);       // W8^2 = -i
+    v[7] = cmul(v[7], make_float2(-R2, -R2));  // W8^3
+    dft4(v[0], v[1], v[2], v[3]);
+    dft4(v[4], v[5], v[6], v[7]);
+}
+
+// frequency held by register r after dft8: k1 + 2 k2 with r = 4 k1 + k2
+__device__ constexpr int dig(int r) { return (r >> 2) + 2 * (r & 3); }
+
+constexpr int LB_T = 512;  // threads of a lag FFT work group (8 points each)
+
+__device__ inline uint32_t lpad8(uint32_t i) { return i + (i >> 3); }
+
+// One radix-8 Stockham pass of stride S over the LDS (S > 1): read the
+// thread's 8 inputs in[j + 512 k], twiddle, DFT, and write
+// out[(j / S) 8 S + j % S + S k].
+template <uint32_t S>
+__device__ inline void pass8(float2 (&v)[8], float2* lds, const float2* T, bool write) {
+    const uint32_t j = threadIdx.x;
 #pragma unroll
-    for (int k1 = 0; k1 < 4; k1++) {
-        float2 a0 = v[4 * k1], a1 = v[4 * k1 + 1], a2 = v[4 * k1 + 2], a3 = v[4 * k1 + 3];
-        dft4(a0, a1, a2, a3);
-        w[k1] = a0;
-        w[k1 + 4] = a1;
-        w[k1 + 8] = a2;
-        w[k1 + 12] = a3;
+    for (int k = 0; k < 8; k++) v[k] = lds[lpad8(j + LB_T * k)];
+    __syncthreads();
+#pragma unroll
+    for (int k = 1; k < 8; k++) v[k] = cmul(v[k], T[((j % S) * k * (LB_N / (8 * S))) & (LB_N - 1)]);
+    dft8(v);
+    if (write) {
+        const uint32_t base = (j / S) * 8 * S + (j % S);
+#pragma unroll
+        for (int r = 0; r < 8; r++) lds[lpad8(base + S * dig(r))] = v[r];
+        __syncthreads();
     }
 }
 
-// Forward 4096-point FFT: on entry thread j holds v[k] = z[j + 256 k]; on
-// exit it holds w[k] = Z[j + 256 k] and, when `publish`, the LDS holds Z in
-// natural order (padded indices).  T[m] = exp(-2 pi i m / 4096) in FP32.
-__device__ inline void fft4096(float2 (&v)[16], float2 (&w)[16], float2* lds, const float2* T,
-                               bool publish) {
+// Forward 4096-point FFT in place (four radix-8 Stockham passes): on entry
+// thread j holds v[k] = z[j + 512 k]; on exit register r holds
+// Z[j + 512 dig(r)] and, when `publish`, the LDS holds Z in natural order.
+// T[m] = exp(-2 pi i m / 4096).
+__device__ inline void fft4096(float2 (&v)[8], float2* lds, const float2* T, bool publish) {
     const uint32_t j = threadIdx.x;
-    // pass 0: stride 1, no twiddles
-    dft16(v, w);
+    dft8(v);  // pass 0: stride 1, no twiddles
 #pragma unroll
-    for (int k = 0; k < 16; k++) lds[lpad(j * 16 + k)] = w[k];
+    for (int r = 0; r < 8; r++) lds[lpad8(j * 8 + dig(r))] = v[r];
     __syncthreads();
-    // pass 1: stride 16
-#pragma unroll
-    for (int k = 0; k < 16; k++) v[k] = lds[lpad(j + 256 * k)];
-    __syncthreads();
-#pragma unroll
-    for (int k = 1; k < 16; k++) v[k] = cmul(v[k], T[((j & 15) * k * 16) & (LB_N - 1)]);
-    dft16(v, w);
-    {
-        const uint32_t base = (j >> 4) * 256 + (j & 15);
-#pragma unroll
-        for (int k = 0; k < 16; k++) lds[lpad(base + 16 * k)] = w[k];
-    }
-    __syncthreads();
-    // pass 2: stride 256; output index j + 256 k (natural order)
-#pragma unroll
-    for (int k = 0; k < 16; k++) v[k] = lds[lpad(j + 256 * k)];
-    __syncthreads();
-#pragma unroll
-    for (int k = 1; k < 16; k++) v[k] = cmul(v[k], T[(j * k) & (LB_N - 1)]);
-    dft16(v, w);
-    if (publish) {
-#pragma unroll
-        for (int k = 0; k < 16; k++) lds[lpad(j + 256 * k)] = w[k];
-        __syncthreads();
-    }
+    pass8<8>(v, lds, T, true);
+    pass8<64>(v, lds, T, true);
+    pass8<512>(v, lds, T, publish);  // output index j + 512 k: natural order
 }
 
 // FP32 two-sum: s + v = t + e exactly; the error goes into c
@@ -1460,81 +1455,98 @@ struct LagArgs {
     uint32_t* inexact;      // per row: set when some |H| >= 2^24 (not exact in FP32)
 };
 
-// The spectrum of block b's autocorrelation product conj(X) Y for the
-// thread's 16 frequencies, from Z = FFT(x + i y) held in w (own) and LDS
-// (partners N - k).
-__device__ inline void lag_spectrum(const float2 (&w)[16], const float2* lds, float2 (&P)[16]) {
+// Block b's autocorrelation spectrum conj(X) Y at the thread's 8
+// frequencies k = j + 512 q, from Z = FFT(x + i y) (own values in the
+// registers, partners N - k in the LDS).  W[q] = conj(X) Y for the first
+// block of a pair; the second adds i conj(X) Y.
+template <bool SECOND>
+__device__ inline void lag_spectrum(const float2 (&v)[8], const float2* lds, float2 (&W)[8]) {
     const uint32_t j = threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const uint32_t k = j + 256 * q;
-        const float2 a = w[q];
-        const float2 bb = cconj(lds[lpad((LB_N - k) & (LB_N - 1))]);
-        const float2 sx = cadd(a, bb), dy = csub(a, bb);
+    for (int r = 0; r < 8; r++) {
+        const int q = dig(r);
+        const uint32_t k = j + LB_T * q;
+        const float2 a = v[r];
+        const float2 bb = cconj(lds[lpad8((LB_N - k) & (LB_N - 1))]);
         // conj(X) Y = conj(a + bb) (a - bb) (-i) / 4
-        const float2 pr = cmul(cconj(sx), dy);
-        P[q] = make_float2(0.25f * pr.y, -0.25f * pr.x);
+        const float2 pr = cmul(cconj(cadd(a, bb)), csub(a, bb));
+        const float2 P = make_float2(0.25f * pr.y, -0.25f * pr.x);
+        if (SECOND) W[q] = make_float2(W[q].x - P.y, W[q].y + P.x);  // + i P
+        else W[q] = P;
     }
 }
 
-__device__ inline void lag_load(const LagArgs& a, const int32_t* row, int64_t c, uint32_t blk,
-                                float2 (&v)[16], uint32_t& bad) {
+// The coverage of block blk (4096 bins from blk * HOP; past the row's end
+// and past the last block: c, i.e. H = 0), loaded ahead of its FFT.
+__device__ inline void lag_fetch(const LagArgs& a, const int32_t* row, int32_t c, uint32_t blk,
+                                 uint32_t nblk, int32_t (&g)[8]) {
     const uint32_t j = threadIdx.x;
     const uint64_t t0 = (uint64_t)blk * LB_HOP;
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const uint32_t t = j + 256 * k;
-        const uint64_t gi = t0 + t;
-        float h = 0.f;
-        if (gi < a.n) {
-            const int64_t hv = (int64_t)row[gi] - c;
-            bad |= (hv >= (1 << 24) || hv <= -(1 << 24)) ? 1u : 0u;
-            h = (float)hv;
-        }
-        v[k] = make_float2(t < LB_HOP ? h : 0.f, h);
+    for (int k = 0; k < 8; k++) {
+        const uint64_t gi = t0 + j + LB_T * k;
+        g[k] = (blk < nblk && gi < a.n) ? row[gi] : c;
     }
 }
 
-__global__ __launch_bounds__(NT) void lag_fft32(LagArgs a) {
-    __shared__ float2 lds[LB_N + LB_N / 16];
+// z = x + i y of a fetched block: y = H, x = H on the first HOP points
+__device__ inline void lag_convert(const int32_t (&g)[8], int32_t c, float2 (&v)[8], uint32_t& bad) {
+    const uint32_t j = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int32_t hv = g[k] - c;
+        bad |= (hv >= (1 << 24) || hv <= -(1 << 24)) ? 1u : 0u;
+        const float h = (float)hv;
+        v[k] = make_float2(j + LB_T * k < LB_HOP ? h : 0.f, h);
+    }
+}
+
+__global__ __launch_bounds__(LB_T) __attribute__((amdgpu_waves_per_eu(4))) void lag_fft32(LagArgs a) {
+    __shared__ float2 lds[LB_N + LB_N / 8];
+    __shared__ float2 tw[LB_N];  // the twiddle table, read 21 times per FFT
     const uint32_t r = blockIdx.y, chunk = blockIdx.x, j = threadIdx.x;
+    for (uint32_t m = j; m < LB_N; m += LB_T) tw[m] = a.T[m];
+    __syncthreads();
     const int32_t* row = a.G + (size_t)r * a.ldg;
-    const int64_t c = a.cen[r];
+    const int32_t c = (int32_t)a.cen[r];
     const uint32_t nblk = (a.n + LB_HOP - 1) / LB_HOP;
-    float s[4] = {0.f, 0.f, 0.f, 0.f}, cc[4] = {0.f, 0.f, 0.f, 0.f};
+    float s[2] = {0.f, 0.f}, cc[2] = {0.f, 0.f};
     uint32_t bad = 0;
     const uint32_t p0 = chunk * LB_PAIRS, p1 = min(a.npairs, p0 + LB_PAIRS);
+    // software pipeline: the next block's coverage is in flight during each FFT
+    int32_t g[8];
+    lag_fetch(a, row, c, 2 * p0, nblk, g);
     for (uint32_t p = p0; p < p1; p++) {
-        float2 v[16], w[16], P1[16], P2[16];
-        lag_load(a, row, c, 2 * p, v, bad);
-        fft4096(v, w, lds, a.T, true);
-        lag_spectrum(w, lds, P1);
+        float2 v[8], W[8];
+        lag_convert(g, c, v, bad);
+        lag_fetch(a, row, c, 2 * p + 1, nblk, g);
+        fft4096(v, lds, tw, true);
+        lag_spectrum<false>(v, lds, W);
         __syncthreads();  // the partner reads are done before the LDS is reused
+        lag_convert(g, c, v, bad);  // a block past the last one is all zero
+        lag_fetch(a, row, c, 2 * p + 2, nblk, g);
         if (2 * p + 1 < nblk) {
-            lag_load(a, row, c, 2 * p + 1, v, bad);
-            fft4096(v, w, lds, a.T, true);
-            lag_spectrum(w, lds, P2);
+            fft4096(v, lds, tw, true);
+            lag_spectrum<true>(v, lds, W);
             __syncthreads();
-        } else {
-#pragma unroll
-            for (int q = 0; q < 16; q++) P2[q] = make_float2(0.f, 0.f);
         }
-        // inverse of P1 + i P2 as conj(FFT(conj(.))): c1 + i c2, times 4096
+        // inverse of W = P1 + i P2 as conj(FFT(conj(W))): c1 + i c2, times 4096
 #pragma unroll
-        for (int q = 0; q < 16; q++) v[q] = make_float2(P1[q].x - P2[q].y, -(P1[q].y + P2[q].x));
-        fft4096(v, w, lds, a.T, false);
-        // lags m = j + 256 q, q < 4: c1[m] = Re / N, c2[m] = Im / N (conj flips Im)
+        for (int q = 0; q < 8; q++) v[q] = cconj(W[q]);
+        fft4096(v, lds, tw, false);
+        // lags j and j + 512 sit in registers 0 and 4 (dig 0 and 1):
+        // c1[m] = Re / N, c2[m] = -Im / N (the conjugate)
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            two_sum_add(s[q], cc[q], w[q].x * (1.0f / LB_N));
-            two_sum_add(s[q], cc[q], -w[q].y * (1.0f / LB_N));
+        for (int k = 0; k < 2; k++) {
+            two_sum_add(s[k], cc[k], v[4 * k].x * (1.0f / LB_N));
+            two_sum_add(s[k], cc[k], -v[4 * k].y * (1.0f / LB_N));
         }
     }
     float* out = a.part + ((size_t)r * a.nchunk + chunk) * LB_LAGS * 2;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        out[2 * (j + 256 * q)] = s[q];
-        out[2 * (j + 256 * q) + 1] = cc[q];
+    for (int q = 0; q < 2; q++) {
+        out[2 * (j + LB_T * q)] = s[q];
+        out[2 * (j + LB_T * q) + 1] = cc[q];
     }
     if (bad) atomicOr(&a.inexact[r], 1u);
 }
@@ -1782,7 +1794,7 @@ void moments_widths(Index& ix, const uint32_t* widths, uint32_t nw, double* mean
             ix.prof.end("m32_coverage", t0, st);
             LagArgs la{D.p, ldg, n, cen.p, T.p, npairs, nchunk, part.p, d_bad.p};
             t0 = ix.prof.begin("m32_fft", st);
-            hipLaunchKernelGGL(lag_fft32, dim3(nchunk, nb), dim3(NT), 0, st, la);
+            hipLaunchKernelGGL(lag_fft32, dim3(nchunk, nb), dim3(LB_T), 0, st, la);
             CHK_HIP(hipGetLastError());
             ix.prof.end("m32_fft", t0, st);
             Lag32Eval ea{D.p, ldg, n, cen.p, sumg.p, part.p, nchunk, d_w.p, nw, out_slot.p,
