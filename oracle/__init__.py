@@ -17,7 +17,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# CHK_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build of tests/sanitize); by default the library beside this file
+_LIB_PATH = os.environ.get("CHK_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 CHROMS24 = [f"chr{i}" for i in range(1, 23)] + ["chrX", "chrY"]  # src/bench.rs:15-19
