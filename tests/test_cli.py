@@ -192,3 +192,81 @@ def test_montecarlo_trials_match_oracle(orc, tmp_path, mode, wl):
         got = _read_trials(d) if d.exists() else []
         assert got == trials, k
     assert sum(len(t) for _, b in batches for t in b) > 0
+
+
+HG38_SIZES = {"chr1": 248_956_422, "chr2": 242_193_529, "chr10": 133_797_422, "chr21": 46_709_983,
+              "chrY": 57_227_415}
+
+
+def _mc_dry(tmp_path, rows, mode, trials=60, wl_rows=None):
+    q = tmp_path / "q.bed"
+    q.write_text("".join(f"{c}\t{s}\t{e}\n" for c, s, e in rows))
+    args = ["montecarlo", "-d", str(tmp_path / "nodb"), "-q", str(q), "-t", str(trials), "-b", "16",
+            "-s", "99", "--mode", mode, "--keep-trials", str(tmp_path / "keep"), "--dry-run"]
+    if wl_rows:
+        w = tmp_path / "wl.bed"
+        w.write_text("".join(f"{c}\t{s}\t{e}\n" for c, s, e in wl_rows))
+        args += ["-w", str(w)]
+    run(*args)
+    out = []
+    for b in sorted(os.listdir(tmp_path / "keep")):
+        out += _read_trials(tmp_path / "keep" / b)
+    return out
+
+
+def test_montecarlo_circular_shift_properties(tmp_path):
+    # properties of src/bin/montecarlo.rs:79-122 that hold whatever the draws:
+    # a trial is the query moved by ONE shift on every chromosome: each
+    # original interval whose image meets [0, size) appears once, clipped to
+    # it (so unclipped ones keep their length and their gaps), the others
+    # vanish; chromosomes outside hg38 never appear
+    rows = [("chr1", 1_000, 5_000), ("chr1", 20_000, 20_100), ("chr1", 7_000_000, 7_500_000),
+            ("chr21", 3_000, 4_000), ("chr21", 40_000_000, 40_000_010), ("chrUn", 5, 50),
+            ("chrY", 1_000_000, 1_000_300)]
+    trials = _mc_dry(tmp_path, rows, "circular")
+    assert len(trials) > 30
+    orig = [(c, a, b) for c, a, b in rows if c in HG38_SIZES]
+
+    def image(sh):
+        out = []
+        for c, a, b in orig:
+            lo, hi = max(0, a + sh), min(HG38_SIZES[c], b + sh)
+            if lo < hi:
+                out.append((c, lo, hi))
+        return sorted(out)
+
+    for t in trials:
+        c0, s0, e0 = t[0]
+        cands = {s0 - a for c, a, b in orig if c == c0} | {e0 - b for c, a, b in orig if c == c0}
+        assert any(image(sh) == sorted(t) for sh in cands), t
+
+
+def test_montecarlo_shuffle_properties(tmp_path):
+    # src/bin/montecarlo.rs:124-163: every interval no longer than its
+    # chromosome is placed once per trial, with its length, inside
+    # [0, size); chromosomes outside hg38 are dropped
+    rows = [("chr2", 10, 110), ("chr2", 500, 50_500), ("chr2", 9, 10), ("chr10", 0, 1_000_000),
+            ("chrUn", 1, 2), ("chr21", 46_000_000, 46_709_983)]
+    trials = _mc_dry(tmp_path, rows, "shuffle")
+    want = sorted((c, e - s) for c, s, e in rows if c in HG38_SIZES)
+    assert len(trials) == 60
+    for t in trials:
+        assert sorted((c, e - s) for c, s, e in t) == want
+        for c, s, e in t:
+            assert 0 <= s < e <= HG38_SIZES[c]
+    # placements vary from trial to trial
+    assert len({tuple(t) for t in trials}) == 60
+
+
+def test_montecarlo_whitelist_clips_into_regions(tmp_path):
+    # with a whitelist every trial interval lies inside one whitelist region
+    # of its chromosome (src/bin/montecarlo.rs:71-77)
+    rows = [("chr1", 1_000, 90_000), ("chr1", 5_000_000, 5_100_000), ("chr2", 0, 10_000)]
+    wl = [("chr1", 0, 2_000_000), ("chr1", 60_000_000, 61_000_000), ("chr2", 100, 200_000_000)]
+    for mode in ("circular", "shuffle"):
+        (tmp_path / mode).mkdir()
+        trials = _mc_dry(tmp_path / mode, rows, mode, wl_rows=wl)
+        assert trials
+        for t in trials:
+            for c, s, e in t:
+                assert any(c == wc and ws <= s < e <= we for wc, ws, we in wl), (mode, c, s, e)
