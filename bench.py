@@ -520,7 +520,9 @@ def c5_leg(args, parity):
     ix.profile(False)
     k_ms = ms_k / max(n_k, 1)
     alg = 12 * D + 8 * len(qs) + 16 * F
+    rc_ms, rc_bytes = ix.calibrate_read(3)
     out = {"db_files": F, "db_intervals": D, "index_device_bytes": info["device_bytes"],
+           "read_ceiling_gbs": rc_bytes / (rc_ms * 1e-3) / 1e9,
            "index_build_s": t_index,
            "plain": {"query_intervals": len(qs), "ms_per_step": ms_step,
                      "swept_per_s": (len(qs) + D) / (ms_step * 1e-3), "sweep_kernel_ms": k_ms,
@@ -763,6 +765,11 @@ def main():
                  "gbs_rw": 2 * args.calibrate_bytes / (ms_c * 1e-3) / 1e9}
         log(f"calibration copy: {calib}")
 
+    rc_ms, rc_bytes = ix.calibrate_read(3)
+    read_ceiling = {"bytes": rc_bytes, "ms": rc_ms, "gbs": rc_bytes / (rc_ms * 1e-3) / 1e9,
+                    "kernel": "calib_read_kernel: 16-byte loads of the dense streams only, best of "
+                              "3 runs at each of 3 grid sizes"}
+    log(f"read ceiling of the dense streams: {read_ceiling['gbs']:.0f} GB/s")
     swept = (Q + D) / (ms_step * 1e-3)
     n_sweep, ms_sweep = kern["sweep"]
     sweep_ms = ms_sweep / max(n_sweep, 1)
@@ -784,7 +791,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(args.config),
                      "kernel": "sweep_kernel", "kernel_ms": sweep_ms,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     "read_ceiling_gbs": read_ceiling["gbs"],
+                     "frac_of_read_ceiling": achieved / read_ceiling["gbs"]},
+        "read_ceiling": read_ceiling,
         "kernels_ms_per_step": {k: (v[1] / max(v[0], 1)) for k, v in kern.items()},
         "index_build_s": t_index,
         "tuning": args.tune,

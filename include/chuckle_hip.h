@@ -393,6 +393,12 @@ int chk_server_free(chk_server* s);
 int chk_sweep_noalloc(chk_index* idx, uint64_t n, const uint32_t* shard, const uint32_t* start,
                       const uint32_t* end, uint64_t* d_counts, uint64_t* d_bp);
 
+/* Read-only ceiling of the sweep's input: a streaming kernel that reads the
+ * index's dense streams (int32 start / end coordinates, u16 file ids: what
+ * the dense sweep reads per query file) once with 16-byte loads and nothing
+ * else; best time over reps runs at three grid sizes, and the bytes read. */
+int chk_calibrate_read(chk_index* idx, uint32_t reps, double* best_ms, uint64_t* bytes);
+
 /* Launch a plain device copy kernel ("calib_copy") that reads and writes
  * `bytes` bytes once; used to scale hardware byte counters. */
 int chk_calibrate_copy(chk_index* idx, uint64_t bytes);

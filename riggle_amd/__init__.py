@@ -69,6 +69,7 @@ def _declare(L):
     L.chk_profile_only.argtypes = [vp, C.c_char_p]
     L.chk_profile_read.argtypes = [vp, C.c_char_p, C.POINTER(u64), C.POINTER(C.c_double)]
     L.chk_calibrate_copy.argtypes = [vp, u64]
+    L.chk_calibrate_read.argtypes = [vp, u32, C.POINTER(C.c_double), C.POINTER(u64)]
     L.chk_tune.argtypes = [vp, C.c_char_p, C.c_int64]
     L.chk_index_set_genome.argtypes = [vp, vp]
     L.chk_moments_upload.argtypes = [vp, vp, vp, u64, vp, vp]
@@ -336,6 +337,12 @@ class Index:
     def graph(self, query, d_counts, d_bp, d_p=None, d_llr=None, d_flags=None):
         """Capture the batch's sweep (and p-values) as one replayable HIP graph."""
         return Graph(self, query, d_counts, d_bp, d_p, d_llr, d_flags)
+
+    def calibrate_read(self, reps=3):
+        """(best ms, bytes) of a read-only pass over the dense streams."""
+        ms, by = C.c_double(), C.c_uint64()
+        _check(lib().chk_calibrate_read(self._h, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def calibrate_copy(self, nbytes):
         _check(lib().chk_calibrate_copy(self._h, nbytes))

@@ -767,6 +767,14 @@ int chk_sweep_noalloc(chk_index* idx, uint64_t n, const uint32_t* shard, const u
     });
 }
 
+int chk_calibrate_read(chk_index* idx, uint32_t reps, double* best_ms, uint64_t* bytes) {
+    return guarded([&] {
+        CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");
+        set_device(idx->ix.device);
+        chk::calibrate_read(idx->ix, reps, best_ms, bytes);
+    });
+}
+
 int chk_calibrate_copy(chk_index* idx, uint64_t bytes) {
     return guarded([&] {
         CHK_REQUIRE(idx, CHK_ERR_INVALID, "NULL index");

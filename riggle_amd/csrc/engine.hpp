@@ -330,5 +330,8 @@ void query_reserve(Query& q, uint64_t n, uint32_t files);
 void plan_reserve(Query& q, uint64_t n, uint32_t files);
 // Plain copy of `bytes` (reads and writes that many bytes once).
 void calibrate_copy(Index& ix, uint64_t bytes);
+// Read-only streaming of the index's dense streams (what the dense sweep
+// reads), best time over `reps` runs at three grid sizes.
+void calibrate_read(Index& ix, uint32_t reps, double* best_ms, uint64_t* bytes);
 
 }  // namespace chk
