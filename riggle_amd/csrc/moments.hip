@@ -1399,7 +1399,19 @@ __device__ constexpr int dig(int r) { return (r >> 2) + 2 * (r & 3); }
 
 constexpr int LB_T = 512;  // threads of a lag FFT work group (8 points each)
 
-__device__ inline uint32_t lpad8(uint32_t i) { return i + (i >> 3); }
+#ifndef LB_PADSHIFT
+#define LB_PADSHIFT 5
+#endif
+// one padding slot after every 2^LB_PADSHIFT values (1 in 32 measured best:
+// 101.6 ms for the C3 FFTs against 109 ms at 1 in 8, 113 ms at 1 in 64, 142 ms
+// with none)
+__device__ inline uint32_t lpad8(uint32_t i) { return i + (i >> LB_PADSHIFT); }
+
+// Twiddle tables of the three twiddled passes, laid out so that a wave reads
+// consecutive entries: pass S (8, 64, 512) holds W^(l k 4096 / (8 S)) at
+// tw_off(S) + (k - 1) S + l for l < S, k = 1..7 (4,088 entries in all).
+__device__ constexpr uint32_t tw_off(uint32_t S) { return S == 8 ? 0u : (S == 64 ? 56u : 504u); }
+constexpr uint32_t LB_TW = 7 * (8 + 64 + 512);
 
 // One radix-8 Stockham pass of stride S over the LDS (S > 1): read the
 // thread's 8 inputs in[j + 512 k], twiddle, DFT, and write
@@ -1411,7 +1423,7 @@ __device__ inline void pass8(float2 (&v)[8], float2* lds, const float2* T, bool 
     for (int k = 0; k < 8; k++) v[k] = lds[lpad8(j + LB_T * k)];
     __syncthreads();
 #pragma unroll
-    for (int k = 1; k < 8; k++) v[k] = cmul(v[k], T[((j % S) * k * (LB_N / (8 * S))) & (LB_N - 1)]);
+    for (int k = 1; k < 8; k++) v[k] = cmul(v[k], T[tw_off(S) + (k - 1) * S + (j % S)]);
     dft8(v);
     if (write) {
         const uint32_t base = (j / S) * 8 * S + (j % S);
@@ -1424,7 +1436,7 @@ __device__ inline void pass8(float2 (&v)[8], float2* lds, const float2* T, bool 
 // Forward 4096-point FFT in place (four radix-8 Stockham passes): on entry
 // thread j holds v[k] = z[j + 512 k]; on exit register r holds
 // Z[j + 512 dig(r)] and, when `publish`, the LDS holds Z in natural order.
-// T[m] = exp(-2 pi i m / 4096).
+// T = the twiddled passes' tables (tw_off layout).
 __device__ inline void fft4096(float2 (&v)[8], float2* lds, const float2* T, bool publish) {
     const uint32_t j = threadIdx.x;
     dft8(v);  // pass 0: stride 1, no twiddles
@@ -1449,7 +1461,7 @@ struct LagArgs {
     const int32_t* G;       // coverage rows, row stride ldg
     uint32_t ldg, n;
     const int64_t* cen;     // centring constant per row
-    const float2* T;        // exp(-2 pi i m / 4096), m < 4096
+    const float2* T;        // the passes' twiddle tables (tw_off layout)
     uint32_t npairs, nchunk;
     float* part;            // [row][chunk][LB_LAGS][2]: two-sum (sum, compensation)
     uint32_t* inexact;      // per row: set when some |H| >= 2^24 (not exact in FP32)
@@ -1502,10 +1514,10 @@ __device__ inline void lag_convert(const int32_t (&g)[8], int32_t c, float2 (&v)
 }
 
 __global__ __launch_bounds__(LB_T) __attribute__((amdgpu_waves_per_eu(4))) void lag_fft32(LagArgs a) {
-    __shared__ float2 lds[LB_N + LB_N / 8];
-    __shared__ float2 tw[LB_N];  // the twiddle table, read 21 times per FFT
+    __shared__ float2 lds[LB_N + (LB_N >> LB_PADSHIFT)];
+    __shared__ float2 tw[LB_TW];  // the passes' twiddles, read 21 times per FFT
     const uint32_t r = blockIdx.y, chunk = blockIdx.x, j = threadIdx.x;
-    for (uint32_t m = j; m < LB_N; m += LB_T) tw[m] = a.T[m];
+    for (uint32_t m = j; m < LB_TW; m += LB_T) tw[m] = a.T[m];
     __syncthreads();
     const int32_t* row = a.G + (size_t)r * a.ldg;
     const int32_t c = (int32_t)a.cen[r];
@@ -1715,11 +1727,15 @@ void moments_widths(Index& ix, const uint32_t* widths, uint32_t nw, double* mean
     const uint32_t S = ix.nshards, F = ix.nfiles;
     hipStream_t st = ix.stream;
     const size_t nout = (size_t)F * S * nw;
-    std::vector<float2> th(LB_N);
-    for (uint32_t m = 0; m < LB_N; m++) {
-        const double ang = -2.0 * M_PI * (double)m / (double)LB_N;
-        th[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-    }
+    // the twiddled passes' tables, exp(-2 pi i l k / (8 S)) in FP64 rounded to FP32
+    std::vector<float2> th(LB_TW);
+    for (uint32_t S : {8u, 64u, 512u})
+        for (uint32_t k = 1; k < 8; k++)
+            for (uint32_t l = 0; l < S; l++) {
+                const double ang = -2.0 * M_PI * (double)(l * k) / (double)(8 * S);
+                th[(S == 8 ? 0 : (S == 64 ? 56 : 504)) + (k - 1) * S + l] =
+                    make_float2((float)std::cos(ang), (float)std::sin(ang));
+            }
     DevBuf<float2> T;
     DevBuf<uint32_t> d_w, d_bad;
     DevBuf<double> d_mean, d_var, d_lags;
@@ -1729,13 +1745,13 @@ void moments_widths(Index& ix, const uint32_t* widths, uint32_t nw, double* mean
     DevBuf<float> part;
     DevBuf<uint32_t> redo, redo_rows;
     uint64_t nexact_w = 0;
-    T.reserve(LB_N);
+    T.reserve(LB_TW);
     d_w.reserve(nw);
     d_mean.reserve(nout);
     d_var.reserve(nout);
     if (lags_out) d_lags.reserve((size_t)F * S * LB_LAGS);
     slot_of_fid.reserve(F);
-    CHK_HIP(hipMemcpyAsync(T.p, th.data(), LB_N * sizeof(float2), hipMemcpyHostToDevice, st));
+    CHK_HIP(hipMemcpyAsync(T.p, th.data(), LB_TW * sizeof(float2), hipMemcpyHostToDevice, st));
     CHK_HIP(hipMemcpyAsync(d_w.p, widths, nw * 4, hipMemcpyHostToDevice, st));
     // slots without a table stay NaN
     std::vector<double> nanv(nout, NAN);
