@@ -220,6 +220,7 @@ def c3_leg(ix, cfg, F):
     the 16 widths through the FP32 lag-limited FFT route, timed per kernel;
     the per-width error against the exact-integer oracle comes from the CPU
     leg."""
+    ix.set_genome(np.array([x for _, x in cfg["genome"]], np.uint64))
     ix.profile(True)
     regions = ("m32_coverage", "m32_fft", "m32_eval", "m32_exact")
     for name in regions:
@@ -689,14 +690,14 @@ def main():
     ap.add_argument("--c5-requests", type=int, default=1000,
                     help="requests of the serving leg on the 10,000-file C5 DB")
     ap.add_argument("--legs", default=None,
-                    help="comma list of legs to run (plain,stats,batched,serving,cpu); the "
+                    help="comma list of legs to run (plain,pcie,stats,c3,batched,serving,cpu,c5); the "
                          "plain leg gives the JSON line's value")
     ap.add_argument("--calibrate-bytes", type=int, default=0,
                     help="also run the plain copy kernel over this many bytes (counter scaling)")
     args = ap.parse_args()
     if args.legs is None:  # the serving DB is too large for the CPU and moment legs
         args.legs = ("plain,serving" if args.config == "c5"
-                     else "plain,pcie,stats,batched,serving,cpu,c5")
+                     else "plain,pcie,stats,c3,batched,serving,cpu,c5")
     legs = set(args.legs.split(","))
     if args.config != "c2":
         legs.discard("c5")
@@ -704,6 +705,7 @@ def main():
         legs.discard("cpu")
     if args.skip_stats:
         legs.discard("stats")
+        legs.discard("c3")
     if args.skip_batched:
         legs.discard("batched")
     if args.serve_requests <= 0:
@@ -817,7 +819,8 @@ def main():
         log(f"pcie-inclusive: {rec['pcie_inclusive']['ms_per_step']:.3f} ms/step")
     if "stats" in legs:
         rec["stats"] = stats_leg(ix, qq, cfg, F, args.steps)
-        rec["stats"]["c3_fp32_route"] = c3_leg(ix, cfg, F)
+    if "c3" in legs:
+        rec.setdefault("stats", {})["c3_fp32_route"] = c3_leg(ix, cfg, F)
         log(f"c3 fp32 route: {rec['stats']['c3_fp32_route']['kernels_ms']}")
     if "batched" in legs and args.config == "c2":
         rec["batched"] = batched_leg(ix, cfg, F, nshards, D)
@@ -838,7 +841,7 @@ def main():
         rec_cpu["sample_bp_bit_exact"] = bool((gb[0].astype(np.int64) == xb[0]).all())
         parity["plain_sample_counts"] = rec_cpu["sample_counts_bit_exact"]
         parity["plain_sample_bp"] = rec_cpu["sample_bp_bit_exact"]
-        if "stats" in rec:
+        if hasattr(stats_leg, "outputs"):
             t = time.time()
             rec_cpu["moments_and_pvalues"] = mp = cpu_moments_and_precision(ix, cfg, F, stats_leg.outputs)
             parity["moments_within_1e-6"] = bool(max(mp["moments_rel_err_max"].values()) <= 1e-6)
