@@ -543,8 +543,9 @@ struct SparseArgs {
 // waves; per pair a wave looks up the first DB start of the group's layer
 // that can reach the interval (start >= q.s - lookback + 1) in the jump table
 // and scans the layer's starts up to q.e 64 at a time, loading the next 64
-// starts while the current ones are tested.  Pairs are reduced in an LDS
-// histogram of the group's files, flushed once per item.
+// (start, end, file id) triples while the current ones are tested.  Pairs
+// are reduced in an LDS histogram of the group's files, flushed once per
+// item.
 __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
     __shared__ unsigned long long hc[FG], hb[FG];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
@@ -571,26 +572,36 @@ __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
         uint32_t p = bkt < L.nb ? ufl(a.jt[L.jt + bkt]) : L.n;
         if (p >= L.n) continue;
         const int32_t* ls = a.l_start + L.off;
-        int32_t st = p + lane < L.n ? ls[p + lane] : 0x7FFFFFFF;
+        const int32_t* le = a.l_end + L.off;
+        const uint32_t* lf = a.l_fid + L.off;
+        // a lane's start, end and file id come together, the next 64 of
+        // them in flight while the current ones are tested
+        const bool in0 = p + lane < L.n;
+        int32_t st = in0 ? ls[p + lane] : 0x7FFFFFFF;  // lanes past the segment hold INT_MAX
+        int32_t en = in0 ? le[p + lane] : 0;
+        uint32_t fd = in0 ? lf[p + lane] : 0;
         for (;;) {
-            const bool before_end = st < qe;  // lanes past the segment hold INT_MAX
+            const bool before_end = st < qe;
             const bool more = __ballot(before_end) == ~0ull;
             const uint32_t pn = p + 64;
-            int32_t stn = 0x7FFFFFFF;
-            if (more && pn + lane < L.n) stn = ls[pn + lane];
-            if (before_end && st >= lo) {
-                const uint32_t idx = p + lane;
-                const int32_t en = a.l_end[L.off + idx];
-                if (en > qs) {
-                    const uint32_t f = a.l_fid[L.off + idx] - fbase;
-                    const int64_t ov = (int64_t)min(qe, en) - (int64_t)max(qs, st);
-                    atomicAdd(&hc[f], 1ull);
-                    atomicAdd(&hb[f], (unsigned long long)ov);
-                }
+            int32_t stn = 0x7FFFFFFF, enn = 0;
+            uint32_t fdn = 0;
+            if (more && pn + lane < L.n) {
+                stn = ls[pn + lane];
+                enn = le[pn + lane];
+                fdn = lf[pn + lane];
+            }
+            if (before_end && st >= lo && en > qs) {
+                const uint32_t f = fd - fbase;
+                const int64_t ov = (int64_t)min(qe, en) - (int64_t)max(qs, st);
+                atomicAdd(&hc[f], 1ull);
+                atomicAdd(&hb[f], (unsigned long long)ov);
             }
             if (!more || pn >= L.n) break;
             p = pn;
             st = stn;
+            en = enn;
+            fd = fdn;
         }
     }
     __syncthreads();
