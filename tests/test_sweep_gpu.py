@@ -312,3 +312,37 @@ def test_query_reset_reuses_handle(chk, orc):
         c, bp = ix.sweep(q)
         oc, obp = orc.sweep_ranks(db, qq, F, nq, S)
         assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all(), nq
+
+
+@pytest.mark.parametrize("gap", [3_000_000, 5_000_000, 70_000_000])
+def test_sparse_and_clustered_streams(chk, orc, gap):
+    # DB streams of very different densities, across two file groups: shard 0
+    # spaces its intervals `gap` apart up to the top of the int32 range,
+    # shard 1 mixes a dense 1 Mb cluster with far outliers, shard 2 stays
+    # inside 2 Mb; dense method at two tile sizes against the rank sweep
+    rng = np.random.default_rng(gap % 1_000_003)
+    F, S = 1100, 3
+    n0 = min(400, (2**31 - 200_000) // gap)
+    s0 = (np.arange(n0, dtype=np.int64) * gap + rng.integers(0, 1000, n0)).astype(np.uint32)
+    cl = rng.integers(50_000_000, 51_000_000, 6000).astype(np.uint32)
+    far = rng.integers(0, 2**31 - 200_000, 300).astype(np.uint32)
+    s1 = np.concatenate([cl, far])
+    s2 = rng.integers(0, 2_000_000, 5000).astype(np.uint32)
+    st = np.concatenate([s0, s1, s2])
+    ln = rng.integers(1, 100_000, len(st)).astype(np.uint32)
+    sh = np.concatenate([np.zeros(n0), np.ones(len(s1)), np.full(len(s2), 2)]).astype(np.uint32)
+    fid = rng.integers(0, F, len(st)).astype(np.uint32)
+    db = (sh, st, st + ln, fid)
+    nq = 3
+    qs = np.concatenate([rng.integers(0, 2**31 - 2_100_000, 3000), rng.integers(49_000_000, 52_000_000, 3000),
+                         rng.integers(0, 2_100_000, 3000)]).astype(np.uint32)
+    qsh = np.repeat(np.arange(3), 3000).astype(np.uint32)
+    q = (qsh, qs, qs + rng.integers(1, 2_000_000, len(qs)).astype(np.uint32),
+         rng.integers(0, nq, len(qs)).astype(np.uint32))
+    for items in (4, 8):
+        ix = chk.Index(*db, S, F)
+        ix.tune("sweep.method", 1)
+        ix.tune("sweep.items", items)
+        c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3], nq))
+        oc, obp = orc.sweep_ranks(db, q, F, nq, S)
+        assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all(), items
