@@ -105,8 +105,8 @@ struct LayerSeg {
 };
 
 // HBM-resident index.  Dense form: per (shard, file group) a start stream
-// (int32 start ascending + u16 group-local file id) and an end stream (int32
-// end ascending + u16 id), 12 bytes per interval.  Sparse form: per (shard,
+// (int32 start ascending + u16 group-local file id times 8) and an end stream
+// (int32 end ascending + u16 id times 8), 12 bytes per interval.  Sparse form: per (shard,
 // file group, size layer) the intervals sorted by start as int32 start /
 // int32 end / u32 file id plus a jump table, 12 bytes per interval + tables.
 struct Index {
@@ -124,7 +124,7 @@ struct Index {
     std::vector<uint64_t> str_off;    // [nshards * ngroups + 1] stream segment offsets
     DevBuf<int32_t> s_coord, e_coord;
     DevBuf<int32_t> s_samp, e_samp;   // every PSTRIDE-th stream coordinate (coarse split search)
-    DevBuf<uint16_t> s_lfid, e_lfid;
+    DevBuf<uint16_t> s_lfid, e_lfid;  // group-local file id * 8 (histogram slot byte offset)
     uint32_t nlayers = 0;
     uint64_t span = 0;                // sum over shards of the largest start
     std::vector<uint64_t> layer_n;    // intervals per size layer (all shards)

@@ -52,7 +52,7 @@ __global__ void split_stream_keys(const uint64_t* keys, const uint32_t* vals, si
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     coord[i] = (int32_t)(keys[i] & 0x7FFFFFFFull);
-    lfid[i] = (uint16_t)vals[i];
+    lfid[i] = (uint16_t)(vals[i] * 8u);  // byte offset of the file's u64 histogram slot
 }
 
 // Size class of a length (src/io/meta.rs:33-61 for min 128, factor 4).

@@ -68,6 +68,15 @@ __device__ inline int64_t rdl64(int64_t v, uint32_t l) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 __device__ inline uint32_t popc(uint64_t m) { return (uint32_t)__builtin_popcountll(m); }
+// Copy of a wave-uniform value held in a vector register, so that a 64-bit
+// multiply-add can take it as its addend (an instruction reads at most one
+// scalar register) without a copy per use.
+__device__ inline uint64_t vcopy64(uint64_t s) {
+    uint32_t lo, hi;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(lo) : "s"((uint32_t)s));
+    asm volatile("v_mov_b32 %0, %1" : "=v"(hi) : "s"((uint32_t)(s >> 32)));
+    return ((uint64_t)hi << 32) | lo;
+}
 
 constexpr int BUF_DWORD3 = 0x00020000;  // raw buffer, gfx9-family descriptor word 3
 
@@ -256,6 +265,13 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
         }
     };
 
+    // The streams carry each element's group-local file id times 8: the byte
+    // offset of its u64 slot in hc and hb.
+    auto hist_add = [&](uint32_t fo, unsigned long long cv, unsigned long long bv) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(hc) + fo), cv);
+        atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(hb) + fo), bv);
+    };
+
     // Per-element contribution at job rank P with prefix values (rs, w) at P.
     // E stream: (+rs, x*depth - w); S stream: (rs - P, -(x*depth - w));
     // depth = 2 rs - P.
@@ -266,8 +282,7 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
         const unsigned long long cv =
             (unsigned long long)(m.is_end ? (int64_t)rsv : (int64_t)rsv - (int64_t)P);
         const unsigned long long bv = (unsigned long long)((int64_t)x * (int64_t)dep + wn);
-        atomicAdd(&hc[f], cv);
-        atomicAdd(&hb[f], bv);
+        hist_add(f, cv, bv);
     };
 
     // Contribution of one row at ranks P (per lane) with prefix values (rsv, wq),
@@ -279,8 +294,7 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
         const int64_t wn = IS_END ? -wq : wq;
         const unsigned long long cv =
             (unsigned long long)(IS_END ? (int64_t)rsv : (int64_t)rsv - (int64_t)P);
-        atomicAdd(&hc[f], cv);
-        atomicAdd(&hb[f], (unsigned long long)((int64_t)x * (int64_t)dep + wn));
+        hist_add(f, cv, (unsigned long long)((int64_t)x * (int64_t)dep + wn));
     };
 
     // A full wave span (ITEMS rows of 64 elements) ranked against the register
@@ -295,37 +309,50 @@ __global__ __launch_bounds__(NT, ITEMS <= 8 ? 5 : (ITEMS <= 12 ? 4 : 1)) void sw
         // window's y + b with the coordinates directly (no key per element);
         // past the events, 0xFFFFFFFF maps to 2^31, above every coordinate
         const uint32_t kx = (kv >> 1) + (kv & 1u);
+        constexpr bool IS_END = decltype(end_c)::value;
         uint32_t ce = 0xFFFFFFFFu, rsa = 0;  // prefix values at window event ce
         int64_t wa = 0;
+        // the event-free rows' terms at rank ce: depth (signed for the stream
+        // kind, scalar), count and W term (vector copies)
+        int32_t dep_c = 0;
+        uint64_t cv_c = 0, wn_c = 0;
+        // a row's base rank is the events below its first element, or (after
+        // the first row) the previous row's end rank: events that fall
+        // between two rows are then counted by the row's event loop, where
+        // every lane is past them
+        uint32_t ea = popc(__ballot(kx <= ufl((uint32_t)s.x[0])));
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const bool last = k == ITEMS - 1;
             const uint32_t x = (uint32_t)s.x[k];
-            const uint32_t ea = popc(__ballot(kx <= ufl(x)));
             const uint32_t eb = popc(__ballot(kx <= rdl(x, last ? nvl - 1u : 63u)));
             if (ea != ce) {  // rows without events keep the previous row's prefix
                 ce = ea;
                 rsa = rdl(rv, ea);
                 wa = rdl64(wvv, ea);
+                const int32_t depth = 2 * (int32_t)rsa - (int32_t)(p0 + ea);
+                dep_c = IS_END ? depth : -depth;
+                cv_c = vcopy64(IS_END ? (uint64_t)rsa : (uint64_t)((int64_t)rsa - (int64_t)(p0 + ea)));
+                wn_c = vcopy64(IS_END ? (uint64_t)-wa : (uint64_t)wa);
             }
             if (ea == eb) {
-                if (!last || lane < nvl) add_t(end_c, s.x[k], s.f[k], p0 + ea, rsa, wa);
+                if (!last || lane < nvl)
+                    hist_add(s.f[k], cv_c, (unsigned long long)((int64_t)s.x[k] * (int64_t)dep_c + (int64_t)wn_c));
             } else {
-                // lanes past event t gain its start count (the key's low bit)
-                // and its signed coordinate; events past a lane past none
-                uint32_t c = 0, drs = 0;
-                int64_t dw = 0;
-                for (uint32_t t = ea; t < eb; t++) {
-                    const uint32_t kt = rdl(kv, t);
-                    const uint32_t y = kt >> 1, st = kt & 1u;
-                    const bool past = y + st <= x;
-                    const int64_t wi = st ? (int64_t)y : -(int64_t)y;
-                    c += past ? 1u : 0u;
-                    drs += past ? st : 0u;
-                    dw += past ? wi : 0;
-                }
-                if (!last || lane < nvl) add_t(end_c, s.x[k], s.f[k], p0 + ea + c, rsa + drs, wa + dw);
+                // a lane's rank is ea + c, c = the row's events below it; its
+                // prefix values are the window's at that rank, read across
+                // lanes (ea + c <= eb < 64 on this path)
+                uint32_t c = 0;
+                for (uint32_t t = ea; t < eb; t++) c += rdl(kx, t) <= x ? 1u : 0u;
+                const int src = (int)((ea + c) << 2);
+                const uint32_t rsv = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)rv);
+                const uint32_t wlo = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)wvv);
+                const uint32_t whi =
+                    (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)(uint32_t)((uint64_t)wvv >> 32));
+                const int64_t wq = (int64_t)(((uint64_t)whi << 32) | wlo);
+                if (!last || lane < nvl) add_t(end_c, s.x[k], s.f[k], p0 + ea + c, rsv, wq);
             }
+            ea = eb;
         }
     };
 
