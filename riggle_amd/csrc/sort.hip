@@ -4,7 +4,8 @@
 // the digit-major histogram -> scatter.  The scatter ranks keys stably inside
 // a tile with wave ballots: for each 256-key round a lane finds the lanes of
 // its wave holding the same digit (8 ballots), its rank among them, and the
-// per-wave digit counts are prefix-summed across the 4 waves in LDS.
+// per-wave digit counts are prefix-summed across the 4 waves in LDS.  The
+// event sort's kernels use 1024-thread blocks (see EVT).
 #include "sort.hpp"
 
 namespace chk {
@@ -16,7 +17,7 @@ constexpr int RS_TILE = NT * RS_ITEMS;
 constexpr int SC_ITEMS = 8;        // scan items per thread
 constexpr int SC_TILE = NT * SC_ITEMS;
 
-template <typename T>
+template <typename T, int BT = NT>
 __device__ T block_exclusive_scan(T v, T* lds, T& total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     T x = v;
@@ -29,16 +30,16 @@ __device__ T block_exclusive_scan(T v, T* lds, T& total) {
     __syncthreads();
     if (threadIdx.x == 0) {
         T run = 0;
-        for (int i = 0; i < NT / 64; i++) {
+        for (int i = 0; i < BT / 64; i++) {
             T t = lds[i];
             lds[i] = run;
             run += t;
         }
-        lds[NT / 64] = run;
+        lds[BT / 64] = run;
     }
     __syncthreads();
     T res = x - v + lds[w];
-    total = lds[NT / 64];
+    total = lds[BT / 64];
     __syncthreads();
     return res;
 }
@@ -252,11 +253,15 @@ __global__ __launch_bounds__(NT) void rs_scatter(const uint64_t* kin, const uint
 // LDS digit counters (ballots give the rank among equal digits of one round),
 // and the waves' counters are prefix-summed once per tile.
 
-constexpr int EV_ITEMS = 16;  // keys per thread per tile (2048-key tiles write shorter digit runs: slower)
-constexpr int EV_TILE = NT * EV_ITEMS;
-constexpr int EV_MAXD = 10;
+// 4096-key tiles (2048-key tiles write shorter digit runs: slower) over
+// 1024 threads: each of the 16 waves ranks 256 keys in 4 rounds, so a block's
+// serial ranking chain is a quarter of what 256 threads would take.
+constexpr int EVT = 1024;     // threads per event-sort block
+constexpr int EV_ITEMS = 4;   // keys per thread per tile
+constexpr int EV_TILE = EVT * EV_ITEMS;
+constexpr int EV_MAXD = 9;
 constexpr uint32_t EV_BINS = 1u << EV_MAXD;
-constexpr int EV_WKEYS = EV_TILE / (NT / 64);  // keys per wave and tile
+constexpr int EV_WKEYS = EV_TILE / (EVT / 64);  // keys per wave and tile
 
 struct EvSrc {
     const uint32_t* segid;
@@ -275,12 +280,12 @@ __device__ inline void interval_keys(const EvSrc& s, size_t i, uint64_t& ks, uin
 }
 
 template <bool FIRST>
-__global__ __launch_bounds__(NT) void ev_upsweep(EvSrc src, const uint64_t* keys, size_t m,
+__global__ __launch_bounds__(EVT) void ev_upsweep(EvSrc src, const uint64_t* keys, size_t m,
                                                   int shift, uint32_t nbits, uint32_t* hist,
                                                   uint32_t nblocks) {
     __shared__ uint32_t h[EV_BINS];
     const uint32_t bins = 1u << nbits;
-    for (uint32_t d = threadIdx.x; d < bins; d += NT) h[d] = 0;
+    for (uint32_t d = threadIdx.x; d < bins; d += EVT) h[d] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * EV_TILE + threadIdx.x;
     uint32_t dig[EV_ITEMS];  // all loads issued before the first LDS atomic
@@ -288,7 +293,7 @@ __global__ __launch_bounds__(NT) void ev_upsweep(EvSrc src, const uint64_t* keys
         const size_t n = m / 2, i0 = (size_t)blockIdx.x * (EV_TILE / 2) + threadIdx.x;
 #pragma unroll
         for (int i = 0; i < EV_ITEMS / 2; i++) {
-            const size_t iv = i0 + (size_t)i * NT;
+            const size_t iv = i0 + (size_t)i * EVT;
             uint64_t ks = 0, ke = 0;
             if (iv < n) interval_keys(src, iv, ks, ke);
             dig[2 * i] = iv < n ? (uint32_t)(ks >> shift) & (bins - 1) : 0xFFFFFFFFu;
@@ -297,7 +302,7 @@ __global__ __launch_bounds__(NT) void ev_upsweep(EvSrc src, const uint64_t* keys
     } else {
 #pragma unroll
         for (int i = 0; i < EV_ITEMS; i++) {
-            const size_t idx = base + (size_t)i * NT;
+            const size_t idx = base + (size_t)i * EVT;
             const uint64_t k = idx < m ? keys[idx] : 0ull;
             dig[i] = idx < m ? (uint32_t)(k >> shift) & (bins - 1) : 0xFFFFFFFFu;
         }
@@ -306,24 +311,24 @@ __global__ __launch_bounds__(NT) void ev_upsweep(EvSrc src, const uint64_t* keys
     for (int i = 0; i < EV_ITEMS; i++)
         if (dig[i] != 0xFFFFFFFFu) atomicAdd(&h[dig[i]], 1u);
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < bins; d += NT) hist[(size_t)d * nblocks + blockIdx.x] = h[d];
+    for (uint32_t d = threadIdx.x; d < bins; d += EVT) hist[(size_t)d * nblocks + blockIdx.x] = h[d];
 }
 
 template <bool FIRST, bool LAST>
-__global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin, uint64_t* kout,
+__global__ __launch_bounds__(EVT) void ev_scatter(EvSrc src, const uint64_t* kin, uint64_t* kout,
                                                   size_t m, int shift, uint32_t nbits,
                                                   const uint32_t* offs, uint32_t nblocks,
                                                   uint32_t* key32, uint32_t* rs, int64_t* w) {
-    __shared__ uint32_t cnt[NT / 64][EV_BINS];  // per wave digit counts, then tile-local starts
-    __shared__ uint32_t lbase[EV_BINS];          // first tile-local slot of each digit
-    __shared__ uint32_t gbase[EV_BINS];          // first output position of each digit
-    __shared__ uint64_t stage[EV_TILE];          // the tile in digit order
-    __shared__ uint32_t lsum[NT / 64 + 1];
+    __shared__ uint32_t cnt[EVT / 64][EV_BINS];  // per wave digit counts, then tile-local starts
+    __shared__ uint32_t lbase[EV_BINS];           // first tile-local slot of each digit
+    __shared__ uint32_t gbase[EV_BINS];           // first output position of each digit
+    __shared__ uint64_t stage[EV_TILE];           // the tile in digit order
+    __shared__ uint32_t lsum[EVT / 64 + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t bins = 1u << nbits;
-    for (uint32_t d = tid; d < bins; d += NT) {
+    for (uint32_t d = tid; d < bins; d += EVT) {
 #pragma unroll
-        for (int v = 0; v < NT / 64; v++) cnt[v][d] = 0;
+        for (int v = 0; v < EVT / 64; v++) cnt[v][d] = 0;
     }
     __syncthreads();
     const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -370,23 +375,23 @@ __global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin,
     }
     __syncthreads();
     // tile-local layout: digits in order, within a digit wave 0's keys first
-    const uint32_t dpt = bins > NT ? bins / NT : 1u;  // digits per thread (bins is a power of 2)
+    const uint32_t dpt = bins > EVT ? bins / EVT : 1u;  // digits per thread (bins is a power of 2)
     const uint32_t d0 = tid * dpt;
     uint32_t mine = 0;
     for (uint32_t j = 0; j < dpt; j++)
         if (d0 + j < bins) {
 #pragma unroll
-            for (int v = 0; v < NT / 64; v++) mine += cnt[v][d0 + j];
+            for (int v = 0; v < EVT / 64; v++) mine += cnt[v][d0 + j];
         }
     uint32_t ttot;
-    uint32_t run = block_exclusive_scan<uint32_t>(mine, lsum, ttot);
+    uint32_t run = block_exclusive_scan<uint32_t, EVT>(mine, lsum, ttot);
     for (uint32_t j = 0; j < dpt; j++) {
         const uint32_t d = d0 + j;
         if (d < bins) {
             lbase[d] = run;
             gbase[d] = offs[(size_t)d * nblocks + blockIdx.x];
 #pragma unroll
-            for (int v = 0; v < NT / 64; v++) {
+            for (int v = 0; v < EVT / 64; v++) {
                 const uint32_t t = cnt[v][d];
                 cnt[v][d] = run;
                 run += t;
@@ -405,16 +410,12 @@ __global__ __launch_bounds__(NT) void ev_scatter(EvSrc src, const uint64_t* kin,
     // write out in tile order: runs of one digit go to consecutive positions
     const uint32_t nt = (uint32_t)(m - tile0 < (size_t)EV_TILE ? m - tile0 : (size_t)EV_TILE);
     const uint64_t cmask = (1ull << src.cb) - 1ull;
-    for (uint32_t j = tid; j < nt; j += NT) {
+    for (uint32_t j = tid; j < nt; j += EVT) {
         const uint64_t k = stage[j];
         const uint32_t d = (uint32_t)(k >> shift) & (bins - 1);
         const uint32_t pos = gbase[d] + (j - lbase[d]);
         if (LAST) {
-            const uint32_t lo = (uint32_t)(k & cmask);
-            const int64_t c = (int64_t)(lo >> 1);
-            key32[pos] = lo;
-            rs[pos] = lo & 1u;
-            w[pos] = (lo & 1u) ? c : -c;
+            key32[pos] = (uint32_t)(k & cmask);
         } else {
             kout[pos] = k;
         }
@@ -431,17 +432,30 @@ __global__ void ev_empty(uint32_t* key32, uint32_t* rs, int64_t* w) {
     }
 }
 
-// Joint exclusive scan of rs (u32) and w (i64): tile sums, then the tiles
-// with their bases (reduced in the apply kernel for up to 2048 tiles).
-__global__ __launch_bounds__(NT) void ev_scan_sums(const uint32_t* rs, const int64_t* w, size_t n,
+// The per-event terms of a sorted key 2x + b: rs = b (a start), w = +x for
+// a start and -x for an end.
+__device__ inline void event_terms(const uint32_t (&k)[SC_ITEMS], uint32_t (&va)[SC_ITEMS],
+                                   int64_t (&vb)[SC_ITEMS]) {
+#pragma unroll
+    for (int j = 0; j < SC_ITEMS; j++) {
+        const int64_t c = (int64_t)(k[j] >> 1);
+        va[j] = k[j] & 1u;
+        vb[j] = (k[j] & 1u) ? c : -c;
+    }
+}
+
+// Joint exclusive scan of rs (u32) and w (i64), both derived from the sorted
+// keys: tile sums, then the tiles with their bases (reduced in the apply
+// kernel for up to 2048 tiles).  Positions past n read as key 0 (terms 0).
+__global__ __launch_bounds__(NT) void ev_scan_sums(const uint32_t* key32, size_t n,
                                                     uint32_t* srs, int64_t* sw) {
     __shared__ uint32_t l32[NT / 64 + 1];
     __shared__ int64_t l64[NT / 64 + 1];
     const size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_ITEMS;
-    uint32_t va[SC_ITEMS];
+    uint32_t kk[SC_ITEMS], va[SC_ITEMS];
     int64_t vb[SC_ITEMS];
-    load_run<uint32_t>(rs, base, n, va);
-    load_run<int64_t>(w, base, n, vb);
+    load_run<uint32_t>(key32, base, n, kk);
+    event_terms(kk, va, vb);
     uint32_t a = 0;
     int64_t b = 0;
 #pragma unroll
@@ -460,9 +474,9 @@ __global__ __launch_bounds__(NT) void ev_scan_sums(const uint32_t* rs, const int
 }
 
 template <bool PRE>
-__global__ __launch_bounds__(NT) void ev_scan_apply(uint32_t* rs, int64_t* w, size_t n,
-                                                     const uint32_t* srs, const int64_t* sw,
-                                                     size_t nb) {
+__global__ __launch_bounds__(NT) void ev_scan_apply(const uint32_t* key32, uint32_t* rs, int64_t* w,
+                                                     size_t n, const uint32_t* srs,
+                                                     const int64_t* sw, size_t nb) {
     __shared__ uint32_t l32[NT / 64 + 1];
     __shared__ int64_t l64[NT / 64 + 1];
     uint32_t pa;
@@ -481,10 +495,10 @@ __global__ __launch_bounds__(NT) void ev_scan_apply(uint32_t* rs, int64_t* w, si
         pb = sw[blockIdx.x];
     }
     const size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_ITEMS;
-    uint32_t va[SC_ITEMS];
+    uint32_t kk[SC_ITEMS], va[SC_ITEMS];
     int64_t vb[SC_ITEMS];
-    load_run<uint32_t>(rs, base, n, va);
-    load_run<int64_t>(w, base, n, vb);
+    load_run<uint32_t>(key32, base, n, kk);
+    event_terms(kk, va, vb);
     uint32_t a = 0;
     int64_t b = 0;
 #pragma unroll
@@ -542,15 +556,15 @@ void sort_events(const uint32_t* segid, const uint32_t* start, const uint32_t* e
         const uint64_t* kin = first ? nullptr : buf[(p - 1) & 1];
         uint64_t* kout = last ? nullptr : buf[p & 1];
         if (first)
-            hipLaunchKernelGGL(ev_upsweep<true>, dim3(nblocks), dim3(NT), 0, st, src, kin, m, shift, nb,
+            hipLaunchKernelGGL(ev_upsweep<true>, dim3(nblocks), dim3(EVT), 0, st, src, kin, m, shift, nb,
                                s.hist.p, nblocks);
         else
-            hipLaunchKernelGGL(ev_upsweep<false>, dim3(nblocks), dim3(NT), 0, st, src, kin, m, shift, nb,
+            hipLaunchKernelGGL(ev_upsweep<false>, dim3(nblocks), dim3(EVT), 0, st, src, kin, m, shift, nb,
                                s.hist.p, nblocks);
         exclusive_scan_u32(s.hist.p, (size_t)nblocks << nb, s, st);
         auto scatter = first ? (last ? ev_scatter<true, true> : ev_scatter<true, false>)
                              : (last ? ev_scatter<false, true> : ev_scatter<false, false>);
-        hipLaunchKernelGGL(scatter, dim3(nblocks), dim3(NT), 0, st, src, kin, kout, m, shift, nb,
+        hipLaunchKernelGGL(scatter, dim3(nblocks), dim3(EVT), 0, st, src, kin, kout, m, shift, nb,
                            s.hist.p, nblocks, key32, rs, w);
         CHK_HIP(hipGetLastError());
         shift += (int)nb;
@@ -559,14 +573,14 @@ void sort_events(const uint32_t* segid, const uint32_t* start, const uint32_t* e
     const size_t nbs = (m + SC_TILE - 1) / SC_TILE;
     s.sums.reserve(nbs + 1);
     s.sums64.reserve(nbs + 1);
-    hipLaunchKernelGGL(ev_scan_sums, dim3((unsigned)nbs), dim3(NT), 0, st, rs, w, m, s.sums.p, s.sums64.p);
+    hipLaunchKernelGGL(ev_scan_sums, dim3((unsigned)nbs), dim3(NT), 0, st, key32, m, s.sums.p, s.sums64.p);
     if (nbs <= 2048) {
-        hipLaunchKernelGGL(ev_scan_apply<true>, dim3((unsigned)nbs), dim3(NT), 0, st, rs, w, m, s.sums.p,
+        hipLaunchKernelGGL(ev_scan_apply<true>, dim3((unsigned)nbs), dim3(NT), 0, st, key32, rs, w, m, s.sums.p,
                            s.sums64.p, nbs);
     } else {
         hipLaunchKernelGGL(scan_sums_single<uint32_t>, dim3(1), dim3(NT), 0, st, s.sums.p, nbs);
         hipLaunchKernelGGL(scan_sums_single<int64_t>, dim3(1), dim3(NT), 0, st, s.sums64.p, nbs);
-        hipLaunchKernelGGL(ev_scan_apply<false>, dim3((unsigned)nbs), dim3(NT), 0, st, rs, w, m, s.sums.p,
+        hipLaunchKernelGGL(ev_scan_apply<false>, dim3((unsigned)nbs), dim3(NT), 0, st, key32, rs, w, m, s.sums.p,
                            s.sums64.p, nbs);
     }
     CHK_HIP(hipGetLastError());
