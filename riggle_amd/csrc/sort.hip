@@ -273,10 +273,14 @@ struct EvSrc {
 // The first pass reads each interval once and takes its two events in turn:
 // tile t holds the events of intervals [t * EV_TILE / 2, (t + 1) * EV_TILE / 2)
 // (any fixed order of the input is fine for an LSD sort's first pass).
-__device__ inline void interval_keys(const EvSrc& s, size_t i, uint64_t& ks, uint64_t& ke) {
-    const uint64_t seg = (uint64_t)s.segid[i] << s.cb;
-    ks = seg | (2ull * s.start[i] + 1ull);
-    ke = seg | (2ull * s.end[i]);
+// Kernels load an interval's three fields for all of their intervals first
+// (at clamped indices, so the loads are unconditional), then form the keys:
+// every load is then in flight at once.
+__device__ inline void interval_keys(uint32_t cb, uint32_t segid, uint32_t start, uint32_t end,
+                                     uint64_t& ks, uint64_t& ke) {
+    const uint64_t seg = (uint64_t)segid << cb;
+    ks = seg | (2ull * start + 1ull);
+    ke = seg | (2ull * end);
 }
 
 template <bool FIRST>
@@ -288,28 +292,39 @@ __global__ __launch_bounds__(EVT) void ev_upsweep(EvSrc src, const uint64_t* key
     for (uint32_t d = threadIdx.x; d < bins; d += EVT) h[d] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * EV_TILE + threadIdx.x;
-    uint32_t dig[EV_ITEMS];  // all loads issued before the first LDS atomic
+    // keys past the end are read at a clamped index and counted with an
+    // increment of 0, so loads and counts are unconditional and all loads are
+    // issued before the first LDS atomic
+    uint32_t dig[EV_ITEMS], inc[EV_ITEMS];
     if (FIRST) {
         const size_t n = m / 2, i0 = (size_t)blockIdx.x * (EV_TILE / 2) + threadIdx.x;
+        uint32_t sg[EV_ITEMS / 2], s0[EV_ITEMS / 2], e0[EV_ITEMS / 2];
+#pragma unroll
+        for (int i = 0; i < EV_ITEMS / 2; i++) {
+            const size_t iv = i0 + (size_t)i * EVT, ic = iv < n ? iv : n - 1;
+            sg[i] = src.segid[ic];
+            s0[i] = src.start[ic];
+            e0[i] = src.end[ic];
+        }
 #pragma unroll
         for (int i = 0; i < EV_ITEMS / 2; i++) {
             const size_t iv = i0 + (size_t)i * EVT;
-            uint64_t ks = 0, ke = 0;
-            if (iv < n) interval_keys(src, iv, ks, ke);
-            dig[2 * i] = iv < n ? (uint32_t)(ks >> shift) & (bins - 1) : 0xFFFFFFFFu;
-            dig[2 * i + 1] = iv < n ? (uint32_t)(ke >> shift) & (bins - 1) : 0xFFFFFFFFu;
+            uint64_t ks, ke;
+            interval_keys(src.cb, sg[i], s0[i], e0[i], ks, ke);
+            dig[2 * i] = (uint32_t)(ks >> shift) & (bins - 1);
+            dig[2 * i + 1] = (uint32_t)(ke >> shift) & (bins - 1);
+            inc[2 * i] = inc[2 * i + 1] = iv < n ? 1u : 0u;
         }
     } else {
 #pragma unroll
         for (int i = 0; i < EV_ITEMS; i++) {
             const size_t idx = base + (size_t)i * EVT;
-            const uint64_t k = idx < m ? keys[idx] : 0ull;
-            dig[i] = idx < m ? (uint32_t)(k >> shift) & (bins - 1) : 0xFFFFFFFFu;
+            dig[i] = (uint32_t)(keys[idx < m ? idx : m - 1] >> shift) & (bins - 1);
+            inc[i] = idx < m ? 1u : 0u;
         }
     }
 #pragma unroll
-    for (int i = 0; i < EV_ITEMS; i++)
-        if (dig[i] != 0xFFFFFFFFu) atomicAdd(&h[dig[i]], 1u);
+    for (int i = 0; i < EV_ITEMS; i++) atomicAdd(&h[dig[i]], inc[i]);
     __syncthreads();
     for (uint32_t d = threadIdx.x; d < bins; d += EVT) hist[(size_t)d * nblocks + blockIdx.x] = h[d];
 }
@@ -340,18 +355,23 @@ __global__ __launch_bounds__(EVT) void ev_scatter(EvSrc src, const uint64_t* kin
     if (FIRST) {  // rounds 2q, 2q+1: start and end events of interval iv(q)
         const size_t n = m / 2;
         const size_t i0 = (size_t)blockIdx.x * (EV_TILE / 2) + (size_t)wv * (EV_WKEYS / 2) + lane;
+        uint32_t sg[EV_ITEMS / 2], s0[EV_ITEMS / 2], e0[EV_ITEMS / 2];
 #pragma unroll
         for (int q = 0; q < EV_ITEMS / 2; q++) {
-            const size_t iv = i0 + (size_t)q * 64;
-            key[2 * q] = key[2 * q + 1] = 0;
-            if (iv < n) interval_keys(src, iv, key[2 * q], key[2 * q + 1]);
+            const size_t iv = i0 + (size_t)q * 64, ic = iv < n ? iv : n - 1;
+            sg[q] = src.segid[ic];
+            s0[q] = src.start[ic];
+            e0[q] = src.end[ic];
             val[2 * q] = val[2 * q + 1] = iv < n;
         }
+#pragma unroll
+        for (int q = 0; q < EV_ITEMS / 2; q++)
+            interval_keys(src.cb, sg[q], s0[q], e0[q], key[2 * q], key[2 * q + 1]);
     } else {
 #pragma unroll
         for (int r = 0; r < EV_ITEMS; r++) {  // all loads issued before the ranking rounds
             const size_t idx = w0 + (size_t)r * 64 + lane;
-            key[r] = idx < m ? kin[idx] : 0ull;
+            key[r] = kin[idx < m ? idx : m - 1];
             val[r] = idx < m;
         }
     }
