@@ -20,6 +20,9 @@
 // the sorted events give, at any rank p, rS = # starts, rE = p - rS and
 // W = sum(starts) - sum(ends) of the events ranked before p.
 #include <algorithm>
+#include <thread>
+
+#include "dbfile.hpp"
 
 #include "engine.hpp"
 
@@ -308,27 +311,84 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
 // of method threshold never goes back to the host.
 
 // Canonical layout from arbitrary per-interval file ids (counting sort).
+// Large batches are validated, counted and laid out over host threads: each
+// thread takes a contiguous range of the input and writes its intervals of
+// file r after those of the earlier ranges, so the layout is the sequential
+// one (file order, input order within a file).  The first invalid interval
+// in input order is the one reported.
 void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const uint32_t* end,
                   const uint32_t* qfile) {
     Index& ix = *q.idx;
     const uint64_t n_in = q.n_in;
     const uint32_t S = ix.nshards, R = q.nfiles;
+    // threads: one per 128k intervals, at most the host's, and few enough
+    // that their per-(file, shard) counters stay small
+    const unsigned T = (unsigned)std::max<uint64_t>(
+        1, std::min<uint64_t>({db::host_threads(), n_in >> 17,
+                               (1ull << 24) / std::max<uint64_t>(1, (uint64_t)R * S)}));
+    struct Part {
+        std::vector<uint64_t> fn, fs;  // per file, per (file, shard)
+        std::vector<uint64_t> len;     // summed lengths per file (exact)
+        uint32_t max_end = 0;
+        uint64_t bad = ~0ull;          // first invalid interval of the range
+        int why = 0;
+    };
+    std::vector<Part> parts(T);
+    auto range = [&](unsigned t) {
+        return std::make_pair(n_in * t / T, n_in * (t + 1) / T);
+    };
+    auto count = [&](unsigned t) {
+        Part& P = parts[t];
+        P.fn.assign(R, 0);
+        P.fs.assign((size_t)R * S, 0);
+        P.len.assign(R, 0);
+        const auto [i0, i1] = range(t);
+        for (uint64_t i = i0; i < i1; i++) {
+            const uint32_t r = qfile ? qfile[i] : 0u;
+            const int why = start[i] >= end[i] ? 1 : (end[i] > 0x7FFFFFFFu ? 2 : (r >= R ? 3 : 0));
+            if (why) {
+                P.bad = i;
+                P.why = why;
+                return;
+            }
+            if (shard[i] >= S || !ix.shard_n[shard[i]]) continue;
+            P.fn[r]++;
+            P.max_end = std::max(P.max_end, end[i]);
+            P.len[r] += end[i] - start[i];
+            P.fs[(size_t)r * S + shard[i]]++;
+        }
+    };
+    auto run = [&](auto&& f) {
+        if (T == 1) {
+            f(0u);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < T; t++) th.emplace_back(f, t);
+        for (auto& x : th) x.join();
+    };
+    run(count);
+    for (const Part& P : parts) {
+        if (P.bad == ~0ull) continue;
+        CHK_REQUIRE(P.why != 1, CHK_ERR_INVALID, "query interval start must be less than end");
+        CHK_REQUIRE(P.why != 2, CHK_ERR_INVALID,
+                    "query coordinate exceeds 2^31-1 (device coordinates are int32)");
+        CHK_REQUIRE(false, CHK_ERR_INVALID, "query file id out of range");
+    }
     q.file_n.assign(R, 0);
     q.file_len.assign(R, 0.0);
     q.fs_n.assign((size_t)R * S, 0);
     q.max_end = 0;
-    for (uint64_t i = 0; i < n_in; i++) {
-        CHK_REQUIRE(start[i] < end[i], CHK_ERR_INVALID, "query interval start must be less than end");
-        CHK_REQUIRE(end[i] <= 0x7FFFFFFFu, CHK_ERR_INVALID,
-                    "query coordinate exceeds 2^31-1 (device coordinates are int32)");
-        const uint32_t r = qfile ? qfile[i] : 0u;
-        CHK_REQUIRE(r < R, CHK_ERR_INVALID, "query file id out of range");
-        if (shard[i] >= S || !ix.shard_n[shard[i]]) continue;
-        q.file_n[r]++;
-        q.max_end = std::max(q.max_end, end[i]);
-        q.file_len[r] += (double)(end[i] - start[i]);
-        q.fs_n[(size_t)r * S + shard[i]]++;
+    std::vector<uint64_t> len(R, 0);
+    for (const Part& P : parts) {
+        q.max_end = std::max(q.max_end, P.max_end);
+        for (uint32_t r = 0; r < R; r++) {
+            q.file_n[r] += P.fn[r];
+            len[r] += P.len[r];
+        }
+        for (size_t k = 0; k < (size_t)R * S; k++) q.fs_n[k] += P.fs[k];
     }
+    for (uint32_t r = 0; r < R; r++) q.file_len[r] = (double)len[r];
     q.file_off.assign(R + 1, 0);
     for (uint32_t r = 0; r < R; r++) q.file_off[r + 1] = q.file_off[r] + q.file_n[r];
     q.n = q.file_off[R];
@@ -336,14 +396,26 @@ void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const 
     q.st_shard.reserve(q.n);
     q.st_start.reserve(q.n);
     q.st_end.reserve(q.n);
-    std::vector<uint64_t> fill(q.file_off.begin(), q.file_off.end() - 1);
-    for (uint64_t i = 0; i < n_in; i++) {
-        if (shard[i] >= S || !ix.shard_n[shard[i]]) continue;
-        const uint64_t k = fill[qfile ? qfile[i] : 0u]++;
-        q.st_shard.p[k] = shard[i];
-        q.st_start.p[k] = start[i];
-        q.st_end.p[k] = end[i];
+    // each range's first slot in every file: after the earlier ranges' intervals
+    std::vector<std::vector<uint64_t>> fill(T);
+    {
+        std::vector<uint64_t> run_off(q.file_off.begin(), q.file_off.end() - 1);
+        for (unsigned t = 0; t < T; t++) {
+            fill[t] = run_off;
+            for (uint32_t r = 0; r < R; r++) run_off[r] += parts[t].fn[r];
+        }
     }
+    run([&](unsigned t) {
+        std::vector<uint64_t>& f = fill[t];
+        const auto [i0, i1] = range(t);
+        for (uint64_t i = i0; i < i1; i++) {
+            if (shard[i] >= S || !ix.shard_n[shard[i]]) continue;
+            const uint64_t k = f[qfile ? qfile[i] : 0u]++;
+            q.st_shard.p[k] = shard[i];
+            q.st_start.p[k] = start[i];
+            q.st_end.p[k] = end[i];
+        }
+    });
     query_stage_commit(q);
 }
 

@@ -346,3 +346,28 @@ def test_sparse_and_clustered_streams(chk, orc, gap):
         c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3], nq))
         oc, obp = orc.sweep_ranks(db, q, F, nq, S)
         assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all(), items
+
+
+def test_large_batch_staged_over_host_threads(chk, orc):
+    # 700k query intervals in 5 files (ids interleaved) with some on a shard the
+    # DB lacks: staged over several host threads, the batch gives the oracle's
+    # matrix; the first invalid interval in input order is the one reported
+    rng = np.random.default_rng(77)
+    F, S, nq, n = 30, 4, 5, 700_000
+    db = flatten(random_files(rng, F, 3, (2_000, 20_000), 3_000_000, 5_000))
+    st = rng.integers(0, 3_000_000, n).astype(np.uint32)
+    q = (rng.integers(0, S, n).astype(np.uint32), st, st + rng.integers(1, 5_000, n).astype(np.uint32),
+         rng.integers(0, nq, n).astype(np.uint32))
+    ix = chk.Index(*db, S, F)
+    c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3], nq))
+    keep = q[0] < 3
+    oc, obp = orc.sweep_ranks(db, tuple(a[keep] for a in q), F, nq, S)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+    bad = [a.copy() for a in q]
+    bad[3][500_000] = nq          # file id out of range, later in the input
+    bad[2][300_000] = bad[1][300_000]  # start == end, earlier
+    with pytest.raises(chk.ChuckleError, match="start must be less than end"):
+        ix.query(bad[0], bad[1], bad[2], bad[3], nq)
+    bad[2][300_000] = bad[1][300_000] + 1
+    with pytest.raises(chk.ChuckleError, match="file id out of range"):
+        ix.query(bad[0], bad[1], bad[2], bad[3], nq)
