@@ -22,6 +22,7 @@
 #include <thread>
 #include <unordered_map>
 
+#include "dbfile.hpp"
 #include "engine.hpp"
 
 namespace chk {
@@ -302,36 +303,85 @@ struct Server {
     // Validate and keep one query file (client thread): intervals on shards
     // absent from the DB are dropped here, as by query_upload.
     uint64_t submit(uint64_t n, const uint32_t* shard, const uint32_t* start, const uint32_t* end) {
+        // the request arrives now: its validation and copy count toward its latency
+        const Clock::time_point t_arrive = Clock::now();
         auto r = std::make_unique<Request>();
         const uint32_t S = ix->nshards;
         r->n_given = n;
-        r->fs_n.assign(S, 0);
-        uint64_t kept = 0;
-        for (uint64_t i = 0; i < n; i++) {
-            CHK_REQUIRE(start[i] < end[i], CHK_ERR_INVALID, "query interval start must be less than end");
-            CHK_REQUIRE(end[i] <= 0x7FFFFFFFu, CHK_ERR_INVALID,
+        // large requests are validated and copied over host threads, each a
+        // contiguous range of the input written after the earlier ranges
+        const unsigned T = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(db::host_threads(), n >> 17));
+        struct Part {
+            std::vector<uint64_t> fs;
+            uint64_t kept = 0, len = 0, bad = ~0ull;
+            uint32_t max_end = 0;
+            int why = 0;
+        };
+        std::vector<Part> parts(T);
+        auto range = [&](unsigned t) { return std::make_pair(n * t / T, n * (t + 1) / T); };
+        auto run = [&](auto&& f) {
+            if (T == 1) {
+                f(0u);
+                return;
+            }
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < T; t++) th.emplace_back(f, t);
+            for (auto& x : th) x.join();
+        };
+        run([&](unsigned t) {
+            Part& P = parts[t];
+            P.fs.assign(S, 0);
+            const auto [i0, i1] = range(t);
+            for (uint64_t i = i0; i < i1; i++) {
+                const int why = start[i] >= end[i] ? 1 : (end[i] > 0x7FFFFFFFu ? 2 : 0);
+                if (why) {
+                    P.bad = i;
+                    P.why = why;
+                    return;
+                }
+                if (shard[i] >= S || !ix->shard_n[shard[i]]) continue;
+                P.kept++;
+                P.fs[shard[i]]++;
+                P.max_end = std::max(P.max_end, end[i]);
+                P.len += end[i] - start[i];
+            }
+        });
+        for (const Part& P : parts) {  // the first invalid interval in input order
+            if (P.bad == ~0ull) continue;
+            CHK_REQUIRE(P.why != 1, CHK_ERR_INVALID, "query interval start must be less than end");
+            CHK_REQUIRE(false, CHK_ERR_INVALID,
                         "query coordinate exceeds 2^31-1 (device coordinates are int32)");
-            kept += shard[i] < S && ix->shard_n[shard[i]] ? 1 : 0;
         }
+        r->fs_n.assign(S, 0);
+        uint64_t kept = 0, len = 0;
+        std::vector<uint64_t> off(T);
+        for (unsigned t = 0; t < T; t++) {
+            off[t] = kept;
+            kept += parts[t].kept;
+            len += parts[t].len;
+            r->max_end = std::max(r->max_end, parts[t].max_end);
+            for (uint32_t c = 0; c < S; c++) r->fs_n[c] += parts[t].fs[c];
+        }
+        r->total_len = (double)len;
         r->shard.resize(kept);
         r->start.resize(kept);
         r->end.resize(kept);
-        uint64_t k = 0;
-        for (uint64_t i = 0; i < n; i++) {
-            if (shard[i] >= S || !ix->shard_n[shard[i]]) continue;
-            r->shard[k] = shard[i];
-            r->start[k] = start[i];
-            r->end[k] = end[i];
-            r->fs_n[shard[i]]++;
-            r->max_end = std::max(r->max_end, end[i]);
-            r->total_len += (double)(end[i] - start[i]);
-            k++;
-        }
+        run([&](unsigned t) {
+            uint64_t k = off[t];
+            const auto [i0, i1] = range(t);
+            for (uint64_t i = i0; i < i1; i++) {
+                if (shard[i] >= S || !ix->shard_n[shard[i]]) continue;
+                r->shard[k] = shard[i];
+                r->start[k] = start[i];
+                r->end[k] = end[i];
+                k++;
+            }
+        });
         r->footprint = footprint(kept);
         std::lock_guard<std::mutex> g(mu);
         CHK_REQUIRE(fatal.empty(), CHK_ERR_HIP, "server thread failed: " + fatal);
         r->id = next_id++;
-        r->t_submit = Clock::now();
+        r->t_submit = t_arrive;
         if (!have_first) {
             have_first = true;
             first_submit = r->t_submit;
