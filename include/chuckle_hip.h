@@ -347,11 +347,13 @@ int chk_graph_free(chk_graph* g);
 typedef struct chk_server chk_server;
 int chk_server_create(chk_index* idx, uint32_t slots, uint32_t max_batch_files,
                       uint64_t hbm_budget, int pvalues, chk_server** out);
-/* Queue one query file (copied; returns at once) and get its ticket. */
+/* Queue one query file (validated and copied, over host threads when large;
+ * returns once it is queued) and get its ticket. */
 int chk_server_submit(chk_server* s, uint64_t n, const uint32_t* shard, const uint32_t* start,
                       const uint32_t* end, uint64_t* ticket);
 /* Block until the request is done; outputs are rows of nfiles(index) entries
- * (any may be NULL).  latency_ms = completion - submission. */
+ * (any may be NULL).  latency_ms = completion - the chk_server_submit call
+ * (its validation and copy included). */
 int chk_server_wait(chk_server* s, uint64_t ticket, uint64_t* counts, uint64_t* bp, double* p,
                     double* llr, uint8_t* flags, double* latency_ms);
 /* Latency percentiles (ms) over completed requests; swept_per_s =
