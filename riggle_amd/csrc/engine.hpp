@@ -56,8 +56,10 @@ struct Profiler {
 // carry group-local 16-bit file ids.
 constexpr uint32_t FG = 1024;
 
-// Stride of the stream samples that the merge-path split search starts from.
+// Strides of the stream samples that the merge-path split search starts
+// from: coarse (every PSTRIDE2-th element), then fine (every PSTRIDE-th).
 constexpr uint32_t PSTRIDE = 64;
+constexpr uint32_t PSTRIDE2 = 4096;
 // Size classes of the layered start-sorted table (src/io/meta.rs:33-81,
 // LayerConfig::default_genomic: min_size 128, growth factor 4).
 constexpr uint32_t LAYER_MIN = 128, LAYER_GF = 4;
@@ -123,7 +125,8 @@ struct Index {
     std::vector<uint64_t> shard_n;    // intervals per shard
     std::vector<uint64_t> str_off;    // [nshards * ngroups + 1] stream segment offsets
     DevBuf<int32_t> s_coord, e_coord;
-    DevBuf<int32_t> s_samp, e_samp;   // every PSTRIDE-th stream coordinate (coarse split search)
+    DevBuf<int32_t> s_samp, e_samp;   // every PSTRIDE-th stream coordinate (split search)
+    DevBuf<int32_t> s_samp2, e_samp2; // every PSTRIDE2-th stream coordinate
     DevBuf<uint16_t> s_lfid, e_lfid;  // group-local file id * 8 (histogram slot byte offset)
     uint32_t nlayers = 0;
     uint64_t span = 0;                // sum over shards of the largest start
@@ -140,6 +143,7 @@ struct Index {
     }
     uint64_t device_bytes() const {
         return s_coord.bytes() + e_coord.bytes() + s_samp.bytes() + e_samp.bytes() +
+               s_samp2.bytes() + e_samp2.bytes() +
                s_lfid.bytes() + e_lfid.bytes() +
                l_start.bytes() + l_end.bytes() + l_fid.bytes() + jt.bytes() + d_lseg.bytes();
     }

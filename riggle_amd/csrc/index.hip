@@ -111,10 +111,10 @@ __global__ void build_jump(const LayerSeg* segs, uint32_t nseg, const int32_t* l
     }
 }
 
-// samp[g] = coord[g * PSTRIDE]
-__global__ void sample_stream(const int32_t* coord, size_t ns, int32_t* samp) {
+// samp[g] = coord[g * stride]
+__global__ void sample_stream(const int32_t* coord, size_t ns, uint32_t stride, int32_t* samp) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < ns) samp[g] = coord[g * PSTRIDE];
+    if (g < ns) samp[g] = coord[g * stride];
 }
 
 __global__ void fill_u64(uint64_t* p, size_t n) {
@@ -279,12 +279,17 @@ void index_build(Index& ix, const uint32_t* shard, const uint32_t* start, const 
         hipLaunchKernelGGL(split_stream_keys, dim3(nblk(n)), dim3(NT), 0, st, keys.p, vals.p,
                            (size_t)n, pass == 0 ? ix.s_coord.p : ix.e_coord.p,
                            pass == 0 ? ix.s_lfid.p : ix.e_lfid.p);
-        const size_t ns = (n + PSTRIDE - 1) / PSTRIDE;
+        const size_t ns = (n + PSTRIDE - 1) / PSTRIDE, ns2 = (n + PSTRIDE2 - 1) / PSTRIDE2;
         DevBuf<int32_t>& samp = pass == 0 ? ix.s_samp : ix.e_samp;
+        DevBuf<int32_t>& samp2 = pass == 0 ? ix.s_samp2 : ix.e_samp2;
         samp.reserve(ns + 1);
+        samp2.reserve(ns2 + 1);
         if (ns)
             hipLaunchKernelGGL(sample_stream, dim3(nblk(ns)), dim3(NT), 0, st,
-                               pass == 0 ? ix.s_coord.p : ix.e_coord.p, ns, samp.p);
+                               pass == 0 ? ix.s_coord.p : ix.e_coord.p, ns, PSTRIDE, samp.p);
+        if (ns2)
+            hipLaunchKernelGGL(sample_stream, dim3(nblk(ns2)), dim3(NT), 0, st,
+                               pass == 0 ? ix.s_coord.p : ix.e_coord.p, ns2, PSTRIDE2, samp2.p);
         CHK_HIP(hipGetLastError());
     }
     // layered table + jump tables

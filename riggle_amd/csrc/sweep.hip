@@ -85,13 +85,32 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t by
                                              BUF_DWORD3);
 }
 
+// Narrows the merge-path split of diagonal d, known to lie in [a, z], with the
+// stream's samples S[g] = stream[g * stride] (global stream positions; the
+// job's stream starts at db_off): the split is after position p iff
+// db_key(D[p]) <= Q[d - 1 - p].
+__device__ inline void narrow_split(const int32_t* S, uint64_t stride, uint64_t db_off,
+                                    const uint32_t* Q, uint64_t d, uint64_t& a, uint64_t& z) {
+    const uint64_t g0 = (db_off + a + stride - 1) / stride, g1 = (db_off + z) / stride;
+    uint64_t lo = g0, hi = g1;  // first sample where the split is at or before it
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        const uint64_t p = mid * stride - db_off;
+        if (db_key(S[mid]) <= Q[d - 1 - p]) lo = mid + 1; else hi = mid;
+    }
+    if (lo > g0) a = (lo - 1) * stride - db_off + 1;
+    if (lo < g1) z = lo * stride - db_off;
+}
+
 // Merge-path split of every tile boundary: part[b] = # DB elements among the
 // first d merged items of its job (DB element before event iff dkey <= qkey).
-// The search first narrows the split to PSTRIDE elements on the stream
-// samples (small and cache-resident), then finishes on the stream itself.
+// The search narrows the split to PSTRIDE2 elements on the coarse samples
+// (every PSTRIDE2-th stream element: small enough to stay cached), then to
+// PSTRIDE on the fine ones, then finishes on the stream itself.
 __global__ void partition_kernel(const Job* jobs, uint32_t njobs, uint32_t nbound, uint32_t tile,
                                  const int32_t* s_coord, const int32_t* e_coord,
                                  const int32_t* s_samp, const int32_t* e_samp,
+                                 const int32_t* s_samp2, const int32_t* e_samp2,
                                  const uint32_t* qkeys, uint32_t* part) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbound) return;
@@ -106,19 +125,10 @@ __global__ void partition_kernel(const Job* jobs, uint32_t njobs, uint32_t nboun
     uint64_t total = (uint64_t)J.n_db + J.m;
     uint64_t d = std::min<uint64_t>((uint64_t)(b - J.part0) * tile, total);
     uint64_t a = d > J.m ? d - J.m : 0, z = std::min<uint64_t>(d, J.n_db);
-    if (z - a > 2 * PSTRIDE) {
-        // samples at stream positions p = g * PSTRIDE - db_off inside [a, z)
-        const int32_t* S = J.is_end ? e_samp : s_samp;
-        const uint64_t g0 = (J.db_off + a + PSTRIDE - 1) / PSTRIDE, g1 = (J.db_off + z) / PSTRIDE;
-        uint64_t lo = g0, hi = g1;  // first sample where the split is at or before p
-        while (lo < hi) {
-            const uint64_t mid = (lo + hi) >> 1;
-            const uint64_t p = mid * PSTRIDE - J.db_off;
-            if (db_key(S[mid]) <= Q[d - 1 - p]) lo = mid + 1; else hi = mid;
-        }
-        if (lo > g0) a = (lo - 1) * PSTRIDE - J.db_off + 1;
-        if (lo < g1) z = lo * PSTRIDE - J.db_off;
-    }
+    if (z - a > 2 * PSTRIDE2)
+        narrow_split(J.is_end ? e_samp2 : s_samp2, PSTRIDE2, J.db_off, Q, d, a, z);
+    if (z - a > 2 * PSTRIDE)
+        narrow_split(J.is_end ? e_samp : s_samp, PSTRIDE, J.db_off, Q, d, a, z);
     while (a < z) {
         uint64_t mid = (a + z) >> 1;
         if (db_key(D[mid]) <= Q[d - 1 - mid]) a = mid + 1; else z = mid;
@@ -832,7 +842,8 @@ void sweep_run(Query& q, uint64_t* d_counts, uint64_t* d_bp, hipStream_t st) {
         t0 = ix.prof.begin("partition", st);
         hipLaunchKernelGGL(partition_kernel, dim3((q.nbounds + NT - 1) / NT), dim3(NT), 0, st,
                            q.jobs.p, (uint32_t)q.jobs_host.size(), q.nbounds, q.tile, ix.s_coord.p,
-                           ix.e_coord.p, ix.s_samp.p, ix.e_samp.p, q.key32.p, q.part.p);
+                           ix.e_coord.p, ix.s_samp.p, ix.e_samp.p, ix.s_samp2.p, ix.e_samp2.p,
+                           q.key32.p, q.part.p);
         CHK_HIP(hipGetLastError());
         ix.prof.end("partition", t0, st);
         SweepArgs args{q.jobs.p, q.chunk_job.p, q.part.p, q.ntiles, q.nchunks, q.chunk, F,
