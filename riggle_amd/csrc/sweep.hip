@@ -577,12 +577,12 @@ struct SparseArgs {
 
 // One block per work item (query intervals of one file x one DB file group).
 // The item's (query interval, size layer) pairs are dealt to the block's
-// waves; per pair a wave looks up the first DB start of the group's layer
-// that can reach the interval (start >= q.s - lookback + 1) in the jump table
-// and scans the layer's starts up to q.e 64 at a time, loading the next 64
-// (start, end, file id) triples while the current ones are tested.  Pairs
-// are reduced in an LDS histogram of the group's files, flushed once per
-// item.
+// waves; per pair a wave reads both ends of its candidate range from the
+// jump table (the first start of the group's layer in the bucket of
+// q.s - lookback + 1, and the first in the bucket after q.e - 1's) and scans
+// the range 128 starts a round, the next round's (start, end, file id)
+// triples in flight while the current ones are tested.  Pairs are reduced in
+// an LDS histogram of the group's files, flushed once per item.
 __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
     __shared__ unsigned long long hc[FG], hb[FG];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
@@ -605,40 +605,56 @@ __global__ __launch_bounds__(NT) void sparse_kernel(SparseArgs a) {
         const int32_t qs = (int32_t)ufl(a.start[i]), qe = (int32_t)ufl(a.end[i]);
         const int64_t lo64 = (int64_t)qs - (int64_t)L.lookback + 1;
         const int32_t lo = lo64 > 0 ? (int32_t)lo64 : 0;
-        const uint32_t bkt = (uint32_t)lo >> L.shift;
-        uint32_t p = bkt < L.nb ? ufl(a.jt[L.jt + bkt]) : L.n;
-        if (p >= L.n) continue;
+        // candidates: from the first start in lo's bucket to the first start
+        // in the bucket after q.e - 1's (both jump-table reads in flight at once)
+        const uint32_t bkt = (uint32_t)lo >> L.shift, bkt1 = ((uint32_t)(qe - 1) >> L.shift) + 1u;
+        const uint32_t p0 = bkt < L.nb ? a.jt[L.jt + bkt] : L.n;
+        const uint32_t p1 = bkt1 < L.nb ? a.jt[L.jt + bkt1] : L.n;
+        uint32_t p = ufl(p0);
+        const uint32_t pe = ufl(p1);
+        if (p >= pe) continue;
         const int32_t* ls = a.l_start + L.off;
         const int32_t* le = a.l_end + L.off;
         const uint32_t* lf = a.l_fid + L.off;
-        // a lane's start, end and file id come together, the next 64 of
-        // them in flight while the current ones are tested
-        const bool in0 = p + lane < L.n;
-        int32_t st = in0 ? ls[p + lane] : 0x7FFFFFFF;  // lanes past the segment hold INT_MAX
-        int32_t en = in0 ? le[p + lane] : 0;
-        uint32_t fd = in0 ? lf[p + lane] : 0;
+        // rounds of two 64-start batches: a lane's (start, end, file id) of
+        // both come together, and the next round's are in flight while the
+        // current round's are tested (lanes past the candidates hold start
+        // INT_MAX)
+        auto fetch = [&](uint32_t q, int32_t (&st)[2], int32_t (&en)[2], uint32_t (&fd)[2]) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint32_t k = q + 64u * h + lane;
+                const bool in = k < pe;
+                st[h] = in ? ls[k] : 0x7FFFFFFF;
+                en[h] = in ? le[k] : 0;
+                fd[h] = in ? lf[k] : 0;
+            }
+        };
+        int32_t st[2], en[2];
+        uint32_t fd[2];
+        fetch(p, st, en, fd);
         for (;;) {
-            const bool before_end = st < qe;
-            const bool more = __ballot(before_end) == ~0ull;
-            const uint32_t pn = p + 64;
-            int32_t stn = 0x7FFFFFFF, enn = 0;
-            uint32_t fdn = 0;
-            if (more && pn + lane < L.n) {
-                stn = ls[pn + lane];
-                enn = le[pn + lane];
-                fdn = lf[pn + lane];
+            const uint32_t pn = p + 128;
+            int32_t stn[2] = {0x7FFFFFFF, 0x7FFFFFFF}, enn[2] = {0, 0};
+            uint32_t fdn[2] = {0, 0};
+            if (pn < pe) fetch(pn, stn, enn, fdn);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (st[h] < qe && st[h] >= lo && en[h] > qs) {
+                    const uint32_t f = fd[h] - fbase;
+                    const int64_t ov = (int64_t)min(qe, en[h]) - (int64_t)max(qs, st[h]);
+                    atomicAdd(&hc[f], 1ull);
+                    atomicAdd(&hb[f], (unsigned long long)ov);
+                }
             }
-            if (before_end && st >= lo && en > qs) {
-                const uint32_t f = fd - fbase;
-                const int64_t ov = (int64_t)min(qe, en) - (int64_t)max(qs, st);
-                atomicAdd(&hc[f], 1ull);
-                atomicAdd(&hb[f], (unsigned long long)ov);
-            }
-            if (!more || pn >= L.n) break;
+            if (pn >= pe) break;
             p = pn;
-            st = stn;
-            en = enn;
-            fd = fdn;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                st[h] = stn[h];
+                en[h] = enn[h];
+                fd[h] = fdn[h];
+            }
         }
     }
     __syncthreads();
