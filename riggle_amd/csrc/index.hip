@@ -474,34 +474,78 @@ __global__ void place_kernel(const PlaceFile* files, uint32_t nfiles, uint64_t n
 // Estimated work of the sparse path per query file: for every (interval,
 // layer) the DB starts in [q.s - lookback + 1, q.e) counted from the jump
 // table at bucket granularity, plus a fixed charge per lookup.  The canonical
-// layout groups intervals by file (file_off).
-__global__ void sparse_visits(const uint32_t* shard, const uint32_t* start, const uint32_t* end,
-                              uint64_t n, const uint64_t* file_off, uint32_t nfiles,
-                              const LayerSeg* lseg, uint32_t nsub, const uint32_t* jt,
-                              unsigned long long* visits) {
-    // each thread sums a contiguous run of VR intervals (one file or two),
-    // then a wave whose lanes all ended in the same file adds one total
-    constexpr uint32_t VR = 32;
-    const uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * VR;
-    uint32_t f = 0xFFFFFFFFu;
-    unsigned long long v = 0;
-    if (i0 < n) {
-        uint32_t lo = 0, hi = nfiles;  // file of interval i0: last with file_off <= i0
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (file_off[mid] <= i0) lo = mid; else hi = mid;
+// layout groups intervals by file (file_off).  The estimate only picks
+// between two exact methods, so it is taken over every stride-th interval of
+// a file (at most VSAMP of them) and scaled by the host.
+// Lanes are (sample, file group x layer) pairs: a wave takes ipw samples at a
+// time (64 / nsub of them, or one with lanes looping over the sub-tables
+// when nsub > 64), so a request of a few thousand intervals spreads its
+// lookups over thousands of lanes.  Each block takes a contiguous range of
+// samples; a lane accumulates the visits of the wave's current file in a
+// register, lanes already past that file (a wave straddling a file boundary)
+// add theirs directly, and the waves' sums meet in LDS: about one atomic per
+// block and file.
+constexpr uint64_t VSAMP = 8192;
+
+__device__ inline uint32_t file_of(const uint64_t* off, uint32_t nfiles, uint64_t i) {
+    uint32_t lo = 0, hi = nfiles;  // last file with off <= i (empty files skipped)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// fo = [file_off (R + 1) | samp_off (R + 1) | stride (R)]
+__global__ __launch_bounds__(NT) void sparse_visits(const uint32_t* shard, const uint32_t* start,
+                                                    const uint32_t* end, const uint64_t* fo,
+                                                    uint32_t nfiles, const LayerSeg* lseg,
+                                                    uint32_t nsub, const uint32_t* jt,
+                                                    uint64_t per_block, uint32_t ipw,
+                                                    unsigned long long* visits) {
+    __shared__ uint32_t wf[NT / 64];
+    __shared__ unsigned long long wsum[NT / 64];
+    const uint64_t* file_off = fo;
+    const uint64_t* samp_off = fo + nfiles + 1;
+    const uint64_t* stride = fo + 2 * ((uint64_t)nfiles + 1);
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint32_t jl = nsub <= 64 ? lane / nsub : 0u;  // this lane's sample in a wave's chunk
+    const uint32_t K0 = nsub <= 64 ? lane - jl * nsub : lane;
+    const uint64_t T = samp_off[nfiles];
+    const uint64_t s0 = (uint64_t)blockIdx.x * per_block;
+    const uint64_t s1 = s0 + per_block < T ? s0 + per_block : T;
+    uint32_t fc = 0xFFFFFFFFu;  // the wave's current file (wave-uniform) and its sample range
+    uint64_t fs0 = 0, fend = 0, fbase = 0, fstride = 1;
+    unsigned long long acc = 0;
+    for (uint64_t sb = s0 + (uint64_t)wv * ipw; sb < s1; sb += (uint64_t)(NT / 64) * ipw) {
+        if (fc == 0xFFFFFFFFu || sb >= fend) {
+            if (fc != 0xFFFFFFFFu) {
+                unsigned long long v = acc;
+                for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+                if (lane == 0 && v) atomicAdd(&visits[fc], v);
+                acc = 0;
+            }
+            fc = file_of(samp_off, nfiles, sb);
+            fs0 = samp_off[fc];
+            fend = samp_off[fc + 1];
+            fbase = file_off[fc];
+            fstride = stride[fc];
         }
-        f = lo;
-        const uint64_t i1 = i0 + VR < n ? i0 + VR : n;
-        for (uint64_t i = i0; i < i1; i++) {
-            while (i >= file_off[f + 1]) {  // next non-empty file
-                if (v) atomicAdd(&visits[f], v);
-                v = 0;
-                f++;
+        const uint64_t sj = sb + jl;
+        const bool act = jl < ipw && sj < s1;
+        uint32_t f = fc;
+        unsigned long long v = 0;
+        if (act) {
+            uint64_t i;
+            if (sj < fend) {
+                i = fbase + (sj - fs0) * fstride;
+            } else {
+                f = file_of(samp_off, nfiles, sj);
+                i = file_off[f] + (sj - samp_off[f]) * stride[f];
             }
             const uint32_t s = shard[i];
             const int64_t qs = start[i], qe = end[i];
-            for (uint32_t K = 0; K < nsub; K++) {  // every (file group, layer) of the shard
+            for (uint32_t K = K0; K < nsub; K += 64) {
                 const LayerSeg L = lseg[(size_t)s * nsub + K];
                 if (L.n == 0) continue;
                 const int64_t a = qs - (int64_t)L.lookback + 1;
@@ -512,13 +556,33 @@ __global__ void sparse_visits(const uint32_t* shard, const uint32_t* start, cons
                 v += (j1 > j0 ? j1 - j0 : 0) + SPARSE_UNIT;
             }
         }
+        if (f != fc) {
+            if (v) atomicAdd(&visits[f], v);
+        } else {
+            acc += v;
+        }
     }
-    const uint32_t f0 = __builtin_amdgcn_readfirstlane(f);
-    if (__ballot(f == f0) == __ballot(1)) {
-        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-        if ((threadIdx.x & 63) == 0 && f0 != 0xFFFFFFFFu && v) atomicAdd(&visits[f0], v);
-    } else if (f != 0xFFFFFFFFu && v) {
-        atomicAdd(&visits[f], v);
+    // waves' sums: equal files merged by one thread, in wave order
+    unsigned long long v = acc;
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (lane == 0) {
+        wf[wv] = fc;
+        wsum[wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t f = 0xFFFFFFFFu;
+        unsigned long long sum = 0;
+        for (uint32_t w = 0; w < NT / 64; w++) {
+            if (wf[w] == 0xFFFFFFFFu) continue;
+            if (wf[w] != f) {
+                if (f != 0xFFFFFFFFu && sum) atomicAdd(&visits[f], sum);
+                f = wf[w];
+                sum = 0;
+            }
+            sum += wsum[w];
+        }
+        if (f != 0xFFFFFFFFu && sum) atomicAdd(&visits[f], sum);
     }
 }
 
@@ -558,7 +622,7 @@ void query_reserve(Query& q, uint64_t n, uint32_t files) {
     q.row.reserve(n);
     q.segid.reserve(n);
     q.place.reserve(files);
-    q.vis_off.reserve((size_t)files + 1);
+    q.vis_off.reserve(3 * (size_t)files + 2);
     q.vis.reserve(files);
     q.keys.reserve(2 * n);
     q.key32.reserve(2 * n + 1);
@@ -572,23 +636,46 @@ void query_visits(Query& q) {
     Index& ix = *q.idx;
     const uint32_t R = q.nfiles;
     q.file_visits.assign(R, 0.0);
-    if (!q.n || ix.n == 0) return;
+    const uint32_t nsub = ix.ngroups * ix.nlayers;
+    if (!q.n || ix.n == 0 || nsub == 0) return;
+    // per file: every stride-th interval, at most VSAMP of them
+    std::vector<uint64_t> fo(3 * (size_t)R + 2);
+    uint64_t T = 0;
+    for (uint32_t r = 0; r < R; r++) {
+        const uint64_t n = q.file_off[r + 1] - q.file_off[r];
+        const uint64_t st = std::max<uint64_t>(1, (n + VSAMP - 1) / VSAMP);
+        fo[r] = q.file_off[r];
+        fo[(size_t)R + 1 + r] = T;
+        fo[2 * (size_t)R + 2 + r] = st;
+        T += (n + st - 1) / st;
+    }
+    fo[R] = q.file_off[R];
+    fo[2 * (size_t)R + 1] = T;
     // buffers of the query object: a free here would wait for the whole device
-    DevBuf<uint64_t>& d_off = q.vis_off;
+    DevBuf<uint64_t>& d_fo = q.vis_off;
     DevBuf<uint64_t>& d_vis = q.vis;
-    d_off.reserve(R + 1);
+    d_fo.reserve(fo.size());
     d_vis.reserve(R);
-    CHK_HIP(hipMemcpyAsync(d_off.p, q.file_off.data(), (R + 1) * 8, hipMemcpyHostToDevice, q.stream));
+    CHK_HIP(hipMemcpyAsync(d_fo.p, fo.data(), fo.size() * 8, hipMemcpyHostToDevice, q.stream));
     zero_u64(d_vis.p, R, q.stream);
-    const unsigned g = (unsigned)((q.n + (uint64_t)NT * 32 - 1) / ((uint64_t)NT * 32));
-    hipLaunchKernelGGL(sparse_visits, dim3(g), dim3(NT), 0, q.stream, q.c_shard.p, q.c_start.p,
-                       q.c_end.p, q.n, d_off.p, R, ix.d_lseg.p, ix.ngroups * ix.nlayers, ix.jt.p,
+    const uint32_t ipw = nsub <= 64 ? 64 / nsub : 1u;
+    // contiguous sample ranges, at least one wave chunk per wave, up to eight
+    // blocks per CU
+    const uint64_t chunks = (T + ipw - 1) / ipw;
+    const uint64_t g = std::max<uint64_t>(1, std::min<uint64_t>((chunks + NT / 64 - 1) / (NT / 64),
+                                                                (uint64_t)ix.ncu * 8));
+    const uint64_t per_block = (T + g - 1) / g;
+    hipLaunchKernelGGL(sparse_visits, dim3((unsigned)g), dim3(NT), 0, q.stream, q.c_shard.p,
+                       q.c_start.p, q.c_end.p, d_fo.p, R, ix.d_lseg.p, nsub, ix.jt.p, per_block, ipw,
                        reinterpret_cast<unsigned long long*>(d_vis.p));
     CHK_HIP(hipGetLastError());
     std::vector<uint64_t> h(R);
     CHK_HIP(hipMemcpyAsync(h.data(), d_vis.p, R * 8, hipMemcpyDeviceToHost, q.stream));
-    CHK_HIP(hipStreamSynchronize(q.stream));
-    for (uint32_t r = 0; r < R; r++) q.file_visits[r] = (double)h[r];
+    CHK_HIP(hipStreamSynchronize(q.stream));  // fo and h are host memory of this frame
+    for (uint32_t r = 0; r < R; r++) {
+        const uint64_t ns = fo[(size_t)R + 2 + r] - fo[(size_t)R + 1 + r];
+        if (ns) q.file_visits[r] = (double)h[r] * (double)(q.file_off[r + 1] - q.file_off[r]) / (double)ns;
+    }
 }
 
 void query_place(Query& q) {
