@@ -147,12 +147,29 @@ struct Server {
         q.st_shard.reserve(n);
         q.st_start.reserve(n);
         q.st_end.reserve(n);
-        for (uint32_t b = 0; b < B; b++) {
-            const Request* r = s.reqs[b];
-            const uint64_t o = q.file_off[b], m = r->start.size();
-            std::memcpy(q.st_shard.p + o, r->shard.data(), m * 4);
-            std::memcpy(q.st_start.p + o, r->start.data(), m * 4);
-            std::memcpy(q.st_end.p + o, r->end.data(), m * 4);
+        // staging copy of positions [a, b) of the batch layout; large batches
+        // are split over host threads by position
+        auto stage = [&](uint64_t a, uint64_t e) {
+            uint32_t b = (uint32_t)(std::upper_bound(q.file_off.begin(), q.file_off.end(), a) -
+                                    q.file_off.begin()) - 1;
+            for (; a < e && b < B; b++) {
+                const Request* r = s.reqs[b];
+                const uint64_t o = q.file_off[b], k = a - o;
+                const uint64_t m = std::min<uint64_t>(q.file_off[b + 1], e) - a;
+                if (!m) continue;
+                std::memcpy(q.st_shard.p + a, r->shard.data() + k, m * 4);
+                std::memcpy(q.st_start.p + a, r->start.data() + k, m * 4);
+                std::memcpy(q.st_end.p + a, r->end.data() + k, m * 4);
+                a += m;
+            }
+        };
+        const unsigned T = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(db::host_threads(), n >> 17));
+        if (T == 1) {
+            stage(0, n);
+        } else {
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < T; t++) th.emplace_back(stage, n * t / T, n * (t + 1) / T);
+            for (auto& x : th) x.join();
         }
         query_stage_commit(q);
         if (opt.pvalues) {

@@ -112,3 +112,25 @@ def test_reserved_server_matches_alone(chk, orc, served):
     oc, obp = oracle_row(orc, db, F, files[5])
     assert (ref[5]["counts"].astype(np.int64) == oc).all() and (ref[5]["bp"].astype(np.int64) == obp).all()
     srv.close()
+
+
+def test_large_requests_staged_over_host_threads(chk, orc, served):
+    # batches above 2^18 intervals are copied into the slot's staging by
+    # several host threads, each a range of positions that may cross request
+    # boundaries (an empty request -- every interval on a shard the DB lacks --
+    # included)
+    ix, db, F, _ = served
+    rng = np.random.default_rng(47)
+    sizes = (300_000, 1, 280_000, 70_000)
+    files = [random_files(rng, 1, S, n, SPAN, 20_000)[0] for n in sizes]
+    absent = (np.full(500, S + 2, np.uint32), np.arange(500, dtype=np.uint32),
+              np.arange(500, dtype=np.uint32) + 10)
+    srv = chk.Server(ix, slots=1, max_batch_files=8)
+    order = [files[0], absent, files[1], files[2], files[3]]
+    tickets = [srv.submit(*f) for f in order]
+    rows = [srv.wait(t) for t in tickets]
+    assert (rows[1]["counts"] == 0).all() and (rows[1]["bp"] == 0).all()
+    for r, f in zip(rows[:1] + rows[2:], files):
+        oc, obp = oracle_row(orc, db, F, f)
+        assert (r["counts"].astype(np.int64) == oc).all() and (r["bp"].astype(np.int64) == obp).all()
+    srv.close()
