@@ -29,9 +29,22 @@ namespace chk {
 
 using Clock = std::chrono::steady_clock;
 
+// A request's intervals (shard, start and end of up to cap intervals in one
+// allocation).  Left uninitialised: the submit copy writes every element it
+// keeps.  Buffers go back to the server's spare list when a request
+// completes, so steady serving neither zero-fills nor page-faults them.
+struct IvBuf {
+    std::unique_ptr<uint32_t[]> p;
+    uint64_t cap = 0;
+    uint32_t* shard() const { return p.get(); }
+    uint32_t* start() const { return p.get() + cap; }
+    uint32_t* end() const { return p.get() + 2 * cap; }
+};
+
 struct Request {
     uint64_t id = 0;
-    std::vector<uint32_t> shard, start, end;  // on shards of the DB only
+    IvBuf iv;        // on shards of the DB only
+    uint64_t n = 0;  // intervals kept
     std::vector<uint64_t> fs_n;               // intervals per shard
     double total_len = 0.0;
     uint32_t max_end = 0;
@@ -74,6 +87,8 @@ struct Server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> pending;
+    std::vector<IvBuf> spare;  // interval buffers of completed requests, at most MAX_SPARE
+    static constexpr size_t MAX_SPARE = 8;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     uint64_t next_id = 1, inflight_bytes = 0;
     bool stop = false;
@@ -137,10 +152,10 @@ struct Server {
         for (uint32_t b = 0; b < B; b++) {
             const Request* r = s.reqs[b];
             q.max_end = std::max(q.max_end, r->max_end);
-            q.file_n[b] = r->start.size();
+            q.file_n[b] = r->n;
             q.file_len[b] = r->total_len;
             std::copy(r->fs_n.begin(), r->fs_n.end(), q.fs_n.begin() + (size_t)b * S);
-            q.file_off[b + 1] = q.file_off[b] + r->start.size();
+            q.file_off[b + 1] = q.file_off[b] + r->n;
         }
         const uint64_t n = q.file_off[B];
         q.n = q.n_in = n;
@@ -157,9 +172,9 @@ struct Server {
                 const uint64_t o = q.file_off[b], k = a - o;
                 const uint64_t m = std::min<uint64_t>(q.file_off[b + 1], e) - a;
                 if (!m) continue;
-                std::memcpy(q.st_shard.p + a, r->shard.data() + k, m * 4);
-                std::memcpy(q.st_start.p + a, r->start.data() + k, m * 4);
-                std::memcpy(q.st_end.p + a, r->end.data() + k, m * 4);
+                std::memcpy(q.st_shard.p + a, r->iv.shard() + k, m * 4);
+                std::memcpy(q.st_start.p + a, r->iv.start() + k, m * 4);
+                std::memcpy(q.st_end.p + a, r->iv.end() + k, m * 4);
                 a += m;
             }
         };
@@ -201,6 +216,38 @@ struct Server {
         CHK_HIP(hipEventRecord(s.done, s.stream));
     }
 
+    // Keep a completed request's interval buffer for later requests (lock
+    // held): at most MAX_SPARE of them, the larger ones preferred.
+    void give_back(IvBuf&& b) {
+        if (!b.cap) return;
+        if (spare.size() < MAX_SPARE) {
+            spare.push_back(std::move(b));
+            return;
+        }
+        auto small = std::min_element(spare.begin(), spare.end(),
+                                      [](const IvBuf& x, const IvBuf& y) { return x.cap < y.cap; });
+        if (small->cap < b.cap) *small = std::move(b);
+    }
+
+    // The smallest spare buffer that holds n intervals, or a new one.
+    IvBuf take_buffer(uint64_t n) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            auto best = spare.end();
+            for (auto it = spare.begin(); it != spare.end(); ++it)
+                if (it->cap >= n && (best == spare.end() || it->cap < best->cap)) best = it;
+            if (best != spare.end()) {
+                IvBuf b = std::move(*best);
+                spare.erase(best);
+                return b;
+            }
+        }
+        IvBuf b;
+        b.p.reset(new uint32_t[3 * std::max<uint64_t>(n, 1)]);
+        b.cap = std::max<uint64_t>(n, 1);
+        return b;
+    }
+
     // Copy each request's row out of a completed slot (lock held).
     void complete(Slot& s, const std::string& err) {
         const uint32_t F = ix->nfiles;
@@ -222,9 +269,7 @@ struct Server {
             }
             r->t_done = now;
             r->done = true;
-            std::vector<uint32_t>().swap(r->shard);
-            std::vector<uint32_t>().swap(r->start);
-            std::vector<uint32_t>().swap(r->end);
+            give_back(std::move(r->iv));
         }
         last_done = now;
         inflight_bytes -= s.footprint;
@@ -380,17 +425,19 @@ struct Server {
             for (uint32_t c = 0; c < S; c++) r->fs_n[c] += parts[t].fs[c];
         }
         r->total_len = (double)len;
-        r->shard.resize(kept);
-        r->start.resize(kept);
-        r->end.resize(kept);
+        r->iv = take_buffer(kept);
+        r->n = kept;
+        uint32_t* const ksh = r->iv.shard();
+        uint32_t* const kst = r->iv.start();
+        uint32_t* const ken = r->iv.end();
         run([&](unsigned t) {
             uint64_t k = off[t];
             const auto [i0, i1] = range(t);
             for (uint64_t i = i0; i < i1; i++) {
                 if (shard[i] >= S || !ix->shard_n[shard[i]]) continue;
-                r->shard[k] = shard[i];
-                r->start[k] = start[i];
-                r->end[k] = end[i];
+                ksh[k] = shard[i];
+                kst[k] = start[i];
+                ken[k] = end[i];
                 k++;
             }
         });
