@@ -56,11 +56,12 @@ int chk_device_count(int* count);
  *   start[i] < end[i] <= 2^31-1  -- half-open coordinates
  *   fid[i]   < nfiles   -- DB file id (D_SID)
  * Device layout: per (shard, group of 1024 files), a start stream (int32
- * start ascending + u16 group-local file id) and an end stream (int32 end
- * ascending + u16 id) for the dense merge-path sweep; per (shard, size
- * layer), the intervals sorted by start (int32 start, int32 end, u32 file id)
- * with a jump table for the sparse pair path -- the reference's layer files
- * and jump tables (src/io/layer.rs:34-128, src/io/jump.rs:19-113) in HBM.
+ * start ascending + u16 group-local file id, stored times 8) and an end
+ * stream (int32 end ascending + the same u16) for the dense merge-path
+ * sweep; per (shard, file group, size layer), the intervals sorted by start
+ * (int32 start, int32 end, u32 file id) with a jump table for the sparse pair
+ * path -- the reference's layer files and jump tables
+ * (src/io/layer.rs:34-128, src/io/jump.rs:19-113) in HBM.
  */
 int chk_index_create(int device, uint32_t nshards, uint32_t nfiles, uint64_t n,
                      const uint32_t* shard, const uint32_t* start, const uint32_t* end,
@@ -274,8 +275,10 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
  *   "sweep.sparse_items"   query intervals per sparse work item
  *   "sweep.sparse_max"     largest query file (intervals) swept by the sparse
  *                          pair path; larger files take the dense path
- *                          (0: |DB| / 1024)
- *   "sweep.method"         0 choose per file by size, 1 dense only, 2 sparse only
+ *                          (0: each file's estimated sparse-path work, from
+ *                          the jump tables, against the cost of a DB pass)
+ *   "sweep.method"         0 choose per file (sweep.sparse_max or the work
+ *                          estimate), 1 dense only, 2 sparse only
  *   "moments.fft_mib"      FFT workspace of one launch of the moment build's
  *                          four-step FFT, in MiB (0: the default, 128)
  *   "moments.fft_block"    complex values one four-step row block holds in
@@ -291,8 +294,10 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
  *                          evaluation's prefix scans, a multiple of 4096
  *                          (0: 65536; raised so that a chromosome has at
  *                          most 256 segments)
- * sparse_max and method apply to query handles created afterwards, the
- * moments keys to later moment builds, the others to every later sweep.
+ * The sweep keys apply to every later sweep, chk_query_info and graph
+ * creation (an existing query handle is planned again at its next use); the
+ * moments keys apply to later moment builds.  A graph captured before any
+ * chk_tune call refuses to launch after it.
  */
 int chk_tune(chk_index* idx, const char* key, int64_t value);
 /* Current value of a tuning key (0 = automatic choice). */

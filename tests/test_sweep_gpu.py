@@ -371,3 +371,27 @@ def test_large_batch_staged_over_host_threads(chk, orc):
     bad[2][300_000] = bad[1][300_000] + 1
     with pytest.raises(chk.ChuckleError, match="file id out of range"):
         ix.query(bad[0], bad[1], bad[2], bad[3], nq)
+
+
+def test_auto_plan_with_empty_and_sampled_files(chk, orc):
+    # the per-file visit estimate picks each file's method: files of at most
+    # 8,192 intervals are estimated whole, larger ones from a strided sample;
+    # empty files (no intervals, or all on a shard the DB lacks) sit between
+    # them, and two DB file groups double the sub-tables per interval
+    rng = np.random.default_rng(29)
+    F, S = 1100, 4
+    db = flatten(random_files(rng, F, S, (20, 200), 2_000_000, 5_000))
+    sizes = [0, 9000, 0, 20_000, 5, 8192, 8193, 1, 0]
+    files = [random_files(rng, 1, S, n, 2_000_000, 5_000)[0] for n in sizes]
+    files[0] = (np.full(300, S + 3, np.uint32), np.arange(300, dtype=np.uint32),
+                np.arange(300, dtype=np.uint32) + 7)  # every interval on an absent shard
+    q = flatten(files)
+    ix = chk.Index(*db, S, F)
+    qq = ix.query(q[0], q[1], q[2], q[3], len(sizes))
+    info = qq.info()
+    assert info["nfiles"] == len(sizes) and info["n"] == sum(sizes)
+    c, bp = ix.sweep(qq)
+    keep = q[0] < S
+    oc, obp = orc.sweep_ranks(db, tuple(a[keep] for a in q), F, len(sizes), S)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
+    assert not c[[0, 2, 8]].any() and not bp[[0, 2, 8]].any()
