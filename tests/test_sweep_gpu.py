@@ -395,3 +395,55 @@ def test_auto_plan_with_empty_and_sampled_files(chk, orc):
     oc, obp = orc.sweep_ranks(db, tuple(a[keep] for a in q), F, len(sizes), S)
     assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
     assert not c[[0, 2, 8]].any() and not bp[[0, 2, 8]].any()
+
+
+def _pileup_case(rng):
+    # 40,000 query intervals on one shard, 24,000 of them starting at one
+    # coordinate and 8,000 ending at another: long runs of equal event keys
+    n, S, F = 40_000, 2, 9
+    db = flatten(random_files(rng, F, S, (2000, 6000), 3_000_000, 30_000))
+    s = rng.integers(0, 2_900_000, n).astype(np.uint32)
+    s[:24_000] = 1_234_567
+    e = s + rng.integers(1, 40_000, n).astype(np.uint32)
+    e[30_000:38_000] = 2_950_000
+    s[30_000:38_000] = np.minimum(s[30_000:38_000], 2_949_000)
+    q = (np.zeros(n, np.uint32), s, e, np.zeros(n, np.uint32))
+    return db, q, S, F, 1
+
+
+def _sparse_segments_case(rng):
+    # 40 query files x 25 shards with a few intervals each over 2^31
+    # coordinates: 1,000 event segments above 32-bit coordinate keys, each
+    # sort tile spanning many of them
+    S, F, nq = 25, 6, 40
+    db = flatten(random_files(rng, F, S, (300, 900), 2**31 - 100_000, 50_000))
+    q = flatten(random_files(rng, nq, S, (20, 120), 2**31 - 100_000, 50_000))
+    return db, q, S, F, nq
+
+
+def _hotspot_case(rng):
+    # three dense files with half of their intervals in 1 % of the span
+    S, F, nq = 4, 12, 3
+    db = flatten(random_files(rng, F, S, (5000, 20_000), 20_000_000, 5_000))
+    files = []
+    for _ in range(nq):
+        n = 60_000
+        sh = rng.integers(0, S, n).astype(np.uint32)
+        st = rng.integers(0, 20_000_000, n).astype(np.uint32)
+        hot = rng.random(n) < 0.5
+        st[hot] = 9_000_000 + rng.integers(0, 200_000, int(hot.sum())).astype(np.uint32)
+        files.append((sh, st, st + rng.integers(1, 3_000, n).astype(np.uint32)))
+    return db, flatten(files), S, F, nq
+
+
+@pytest.mark.parametrize("case", [_pileup_case, _sparse_segments_case, _hotspot_case],
+                         ids=["pileup", "sparse_segments", "hotspot"])
+def test_event_sort_skewed_inputs(chk, orc, case):
+    # event distributions that stress the event sort: the dense sweep's
+    # matrices match the oracle
+    db, q, S, F, nq = case(np.random.default_rng(77))
+    ix = chk.Index(*db, S, F)
+    ix.tune("sweep.method", 1)
+    c, bp = ix.sweep(ix.query(q[0], q[1], q[2], q[3] if nq > 1 else None, nq))
+    oc, obp = orc.sweep_ranks(db, q, F, nq, S)
+    assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
