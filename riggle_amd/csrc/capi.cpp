@@ -583,6 +583,9 @@ int chk_tune(chk_index* idx, const char* key, int64_t value) {
         } else if (k == "moments.eval_seg") {
             CHK_REQUIRE(v % 4096 == 0, CHK_ERR_INVALID, "moments.eval_seg must be a multiple of 4096");
             t.eval_seg = v;
+        } else if (k == "moments.fft_pipe") {
+            CHK_REQUIRE(v <= 1, CHK_ERR_INVALID, "moments.fft_pipe must be 0 or 1");
+            t.fft_pipe = v;
         } else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
         idx->ix.tune_version++;
     });
@@ -603,6 +606,7 @@ int chk_tune_get(chk_index* idx, const char* key, int64_t* value) {
         else if (k == "moments.fft_block") *value = t.fft_block;
         else if (k == "moments.round") *value = t.mom_round;
         else if (k == "moments.eval_seg") *value = t.eval_seg;
+        else if (k == "moments.fft_pipe") *value = t.fft_pipe;
         else throw chk::Error(CHK_ERR_INVALID, "unknown tuning key: " + k);
     });
 }

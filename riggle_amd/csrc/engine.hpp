@@ -76,6 +76,7 @@ struct Tuning {
     uint32_t fft_block = 0;      // complex values per four-step row block: 0 (= 4096), 4096 or 8192
     uint32_t mom_round = 0;      // autocorrelation rounding: 0 auto, 1 never, 2 only exact values
     uint32_t eval_seg = 0;       // positions per evaluation segment (multiple of 4096), 0 = default
+    uint32_t fft_pipe = 0;       // forward column pass: 0 persistent pipelined blocks, 1 one block per group
 };
 
 // Compact sliding-window moment tables of the DB files (FP64, device).

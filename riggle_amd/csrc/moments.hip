@@ -476,6 +476,109 @@ __global__ __launch_bounds__(NTF) void fft_cols_fwd(Rows r, uint32_t N1, uint32_
         twa, twb);
 }
 
+// Block barrier that orders LDS traffic only: global loads issued for the
+// next column group stay in flight across it (__syncthreads would wait for
+// them).
+__device__ inline void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+constexpr int BUF_DWORD3 = 0x00020000;  // raw buffer, gfx9-family descriptor word 3
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, BUF_DWORD3);
+}
+
+// Persistent, software-pipelined form of pass 1 for CB * N2 = 8 * blockDim
+// (every thread takes 8 values in the first radix-2^LR pass, whatever N2):
+// block b walks column groups b, b + gridDim, ... (gridDim a multiple of 8,
+// so a group runs on the XCD xcd_block assigns it), and the coverage values
+// of the next group's first pass are loaded into registers while the current
+// group runs its LDS passes and stores.  Buffer loads past a row's n bins
+// (or of a missing odd row) read 0 and need no branch, so the compiler
+// leaves them in flight across the LDS-only barriers.
+template <int LR>
+__global__ __launch_bounds__(NTF) void fft_cols_fwd_p(Rows r, uint32_t N1, uint32_t logN2, uint32_t logCB,
+                                                      uint32_t ngrp) {
+    constexpr int R = 1 << LR, U = 8 / R;  // first-pass radix, units per thread
+    extern __shared__ __align__(16) double2 xs[];
+    __shared__ double2 twa[64], twb[64];
+    load_lds_twiddles(r.tw, twa, twb);
+    __syncthreads();
+    const uint32_t logM = logN2, lognfft = logCB, loghp = logM - LR;
+    const uint32_t ncb = N1 >> logCB, nfm = (1u << lognfft) - 1u, hpm = (1u << loghp) - 1u;
+    // the thread's first-pass units: FFT index f (column c0 + f), k, base
+    uint32_t uf[U], uk[U], ub[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const uint32_t j = threadIdx.x + (uint32_t)u * blockDim.x;
+        uf[u] = j & nfm;
+        const uint32_t jj = j >> lognfft;
+        uk[u] = jj & hpm;
+        ub[u] = ((jj >> loghp) << (loghp + LR)) + uk[u];
+    }
+    uint32_t c[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) c[i] = ((uint32_t)i << loghp) ^ swz_mask((uint32_t)i << loghp);
+    // coverage of group g's first pass: rows 2p and 2p + 1 at n1 + N1 n2
+    auto fetch = [&](uint32_t g, int32_t (&v0)[8], int32_t (&v1)[8]) {
+        const uint32_t bl = xcd_block(g, ngrp), p = bl / ncb, c0 = (bl % ncb) << logCB;
+        const int32_t* g0 = r.G + (size_t)(2 * p) * r.ldg;
+        const bool two = 2 * p + 1 < r.nb;
+        const auto q0 = make_rsrc(g0, r.n * 4u);
+        const auto q1 = make_rsrc(two ? g0 + r.ldg : g0, two ? r.n * 4u : 0u);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const uint32_t idx = c0 + uf[u] + N1 * (ub[u] + ((uint32_t)i << loghp));
+                v0[u * R + i] = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(q0, idx * 4u, 0, 0);
+                v1[u * R + i] = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(q1, idx * 4u, 0, 0);
+            }
+    };
+    const LdsIO L{xs, logM};
+    int32_t a0[8], a1[8];
+    uint32_t g = blockIdx.x;
+    if (g < ngrp) fetch(g, a0, a1);
+    for (; g < ngrp; g += gridDim.x) {
+        const uint32_t bl = xcd_block(g, ngrp), p = bl / ncb, c0 = (bl % ncb) << logCB;
+        const double cen0 = (double)r.cen[2 * p];
+        const double cen1 = 2 * p + 1 < r.nb ? (double)r.cen[2 * p + 1] : 0.0;
+        int32_t b0[8], b1[8];
+        const uint32_t gn = g + gridDim.x;
+        fetch(gn < ngrp ? gn : g, b0, b1);  // past the last group: reloaded, unused
+        // pass 1 from the registers: H = G - c below n bins, 0 past them
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            double2 x[R];
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const uint32_t idx = c0 + uf[u] + N1 * (ub[u] + ((uint32_t)i << loghp));
+                const bool in = idx < r.n;
+                x[i] = {in ? (double)a0[u * R + i] - cen0 : 0.0, in ? (double)a1[u * R + i] - cen1 : 0.0};
+            }
+            butterfly<R, true, false>(x, uk[u], loghp, twa, twb);
+            const uint32_t s0 = lds_slot((uf[u] << logM) + ub[u], logM);
+#pragma unroll
+            for (int i = 0; i < R; i++) xs[s0 ^ c[i]] = x[i];
+        }
+        lds_barrier();
+        for (uint32_t lh = loghp - 3; lh > 0; lh -= 3) {
+            fft_pass<8, true, false, false>(logM, lognfft, lh, L, L, twa, twb);
+            lds_barrier();
+        }
+        double2* X = r.X + (size_t)p * r.N;
+        fft_pass<8, true, false, true>(
+            logM, lognfft, 0, L,
+            [&](uint32_t f, uint32_t pos, double2 v) { X[(c0 + f) + (size_t)N1 * bitrev(pos, logN2)] = v; },
+            twa, twb);
+        lds_barrier();  // the last pass's LDS reads are done before the next group's writes
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            a0[k] = b0[k];
+            a1[k] = b1[k];
+        }
+    }
+}
+
 // Four-step pass 2: the rows k2 and N2 - k2 of one signal pair (row k2 holds
 // Z[k2 + N2 k1], so Z[N - k] of its entry k1 is entry N1 - 1 - k1 of row
 // N2 - k2, which after the DIF sits at the complementary bit-reversed
@@ -1140,6 +1243,9 @@ void moments_build(Index& ix) {
     if (lds_need > 64 * 1024) {
         CHK_HIP(hipFuncSetAttribute((const void*)autocorr_small, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
         CHK_HIP(hipFuncSetAttribute((const void*)fft_cols_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
+        CHK_HIP(hipFuncSetAttribute((const void*)fft_cols_fwd_p<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
+        CHK_HIP(hipFuncSetAttribute((const void*)fft_cols_fwd_p<2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
+        CHK_HIP(hipFuncSetAttribute((const void*)fft_cols_fwd_p<3>, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
         CHK_HIP(hipFuncSetAttribute((const void*)fft_rows, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
         CHK_HIP(hipFuncSetAttribute((const void*)fft_cols_inv, hipFuncAttributeMaxDynamicSharedMemorySize, lds_need));
     }
@@ -1265,8 +1371,18 @@ void moments_build(Index& ix) {
                     rg.cen = cen.p + 2 * p0;
                     rg.A = A.p + (size_t)2 * p0 * n;
                     rg.nb = std::min(2 * pn, nb - 2 * p0);
-                    hipLaunchKernelGGL(fft_cols_fwd, dim3(pn * (N1 / CB)), dim3(tcol), lds_col, st, rg, N1,
-                                       ilog2(N2), ilog2(CB));
+                    const uint32_t ngrp = pn * (N1 / CB), logN2 = ilog2(N2);
+                    if (ix.tune.fft_pipe == 0 && logN2 >= 4 && CB * N2 == 8 * tcol) {
+                        // persistent blocks, one per CU (a multiple of 8 blocks
+                        // when the groups fill the device)
+                        const uint32_t pg = std::min<uint32_t>(ngrp, (uint32_t)ix.ncu);
+                        const uint32_t lr = logN2 % 3 ? logN2 % 3 : 3;
+                        auto k = lr == 1 ? fft_cols_fwd_p<1> : (lr == 2 ? fft_cols_fwd_p<2> : fft_cols_fwd_p<3>);
+                        hipLaunchKernelGGL(k, dim3(pg), dim3(tcol), lds_col, st, rg, N1, logN2, ilog2(CB), ngrp);
+                    } else {
+                        hipLaunchKernelGGL(fft_cols_fwd, dim3(ngrp), dim3(tcol), lds_col, st, rg, N1, logN2,
+                                           ilog2(CB));
+                    }
                     hipLaunchKernelGGL(fft_rows, dim3(pn * (N2 / 2 + 1)), dim3(trow),
                                        2 * N1 * sizeof(double2), st, rg, N1, ilog2(N1), N2);
                     hipLaunchKernelGGL(fft_cols_inv, dim3(pn * (N1 / CB)), dim3(tcol), lds_col, st, rg, N1,
