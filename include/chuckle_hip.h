@@ -294,6 +294,10 @@ int chk_profile_read(chk_index* idx, const char* name, uint64_t* launches, doubl
  *                          evaluation's prefix scans, a multiple of 4096
  *                          (0: 65536; raised so that a chromosome has at
  *                          most 256 segments)
+ *   "moments.fft_pipe"     forward column pass of the four-step FFT: 0
+ *                          persistent blocks that load the next column
+ *                          group while transforming the current one
+ *                          (default); 1 one block per column group
  * The sweep keys apply to every later sweep, chk_query_info and graph
  * creation (an existing query handle is planned again at its next use); the
  * moments keys apply to later moment builds.  A graph captured before any
