@@ -216,3 +216,59 @@ def test_index_open_and_query_open_match_oracle(chk, orc, tmp_path):
             c, b = orc.sweep_bruteforce(db_, (qc.astype(np.uint32), qs, qe, np.zeros(len(qs), np.uint32)),
                                         1, 1)
             assert int(cnt[rows[k], f]) == c[0, 0] and int(bp[rows[k], f]) == b[0, 0]
+
+
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def _counts_by_pair(path):
+    return {(x["query_name"], x["db_name"]): x["overlap_count"] for x in results(path)}
+
+
+# src/tasks/query/mod.rs:455-547: the same DB and query files as the reference's
+# batching tests; half-open overlaps by hand (touching intervals do not count)
+BATCH_DB = {"a.bed": "chr1\t100\t200\nchr1\t300\t400\n", "b.bed": "chr1\t150\t250\n",
+            "c.bed": "chr1\t350\t500\n"}
+BATCH_EXPECTED = {("q1.bed", "a.bed"): 1, ("q1.bed", "b.bed"): 1, ("q2.bed", "a.bed"): 1,
+                  ("q2.bed", "b.bed"): 1, ("q3.bed", "a.bed"): 1, ("q3.bed", "c.bed"): 1,
+                  ("q4.bed", "a.bed"): 2, ("q4.bed", "b.bed"): 1, ("q4.bed", "c.bed"): 1}
+
+
+def test_batch_query_matches_unbatched(tmp_path):
+    src, qdir = tmp_path / "src", tmp_path / "q"
+    src.mkdir()
+    qdir.mkdir()
+    for name, text in BATCH_DB.items():
+        _write(src / name, text)
+    for name, text in (("q1.bed", "chr1\t100\t200\n"), ("q2.bed", "chr1\t200\t350\n"),
+                       ("q3.bed", "chr1\t300\t450\n"), ("q4.bed", "chr1\t50\t600\n")):
+        _write(qdir / name, text)
+    db = str(tmp_path / "db")
+    run("add", "-i", src, "-d", db)
+    got = {}
+    for bs in (0, 1, 3):
+        out = str(tmp_path / f"out_{bs}.json")
+        run("query", "-d", db, "-q", qdir, "-o", out, *(["--batch-size", bs] if bs else []))
+        got[bs] = _counts_by_pair(out)
+    assert got[0] == BATCH_EXPECTED
+    assert got[1] == got[0] and got[3] == got[0]
+
+
+def test_batch_add_matches_unbatched(tmp_path):
+    src, qdir = tmp_path / "src", tmp_path / "q"
+    src.mkdir()
+    qdir.mkdir()
+    for name, text in BATCH_DB.items():
+        _write(src / name, text)
+    _write(qdir / "q.bed", "chr1\t100\t500\n")
+    got = {}
+    for bs in (0, 1):
+        db = str(tmp_path / f"db_{bs}")
+        run("add", "-i", src, "-d", db, *(["--batch-size", bs] if bs else []))
+        out = str(tmp_path / f"out_{bs}.json")
+        run("query", "-d", db, "-q", qdir / "q.bed", "-o", out)
+        got[bs] = _counts_by_pair(out)
+    assert got[0] == {("q.bed", "a.bed"): 2, ("q.bed", "b.bed"): 1, ("q.bed", "c.bed"): 1}
+    assert got[1] == got[0]

@@ -226,3 +226,25 @@ def test_segment_and_chunk_edges_match_oracle(chk, orc):
         assert rows == exact == S * F
         wm, wv = compare(ix, orc, db, F, S, sizes)
         assert wm <= REL and wv <= REL, (seg, wm, wv)
+
+
+def test_fully_covered_chromosome_mean(chk, orc):
+    # src/fourier/moments.rs:580-592: one interval covering chr22 (50,818,468 bp)
+    # gives mean(L) ~ L bins of coverage; the reference checks lookup(500)
+    # within 2 %.  The entry compact_index(500) holds the stored block size
+    # Ls next to it on the x1.01 ladder, and only the window over the partial
+    # last bin (68 bp) falls short, so mean(Ls) = Ls within 1e-8
+    size = 50_818_468
+    one = np.zeros(1, np.uint32)
+    db = (one, one, np.array([size], np.uint32), one)
+    ix, rows, exact = build(chk, db, 1, 1, [size])
+    so, sl, m, v = ix.download_moments()
+    k = orc.compact_index(500)
+    ls = orc.compact_ls((size + 99) // 100)
+    assert so[0] >= 0 and k < sl[0] == len(ls)
+    mean = m[so[0] + k]
+    assert abs(mean - 500.0) / 500.0 < 0.02
+    assert mean == pytest.approx(float(ls[k]), rel=1e-8)
+    assert 0.0 <= v[so[0] + k] < 1e-4
+    worst_m, worst_v = compare(ix, orc, db, 1, 1, [size])
+    assert worst_m <= REL and worst_v <= REL
