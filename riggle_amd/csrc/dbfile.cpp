@@ -13,7 +13,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
 #include <thread>
 #include <unordered_map>
 
@@ -35,23 +39,108 @@ void parallel_for(size_t n, unsigned threads, F&& f) {
         return;
     }
     std::atomic<size_t> next{0};
-    std::vector<std::thread> th;
-    std::vector<std::exception_ptr> err(threads);
-    for (unsigned t = 0; t < threads; t++)
-        th.emplace_back([&, t] {
-            try {
-                for (size_t k; (k = next.fetch_add(1)) < n;) f(k);
-            } catch (...) {
-                err[t] = std::current_exception();
-                next = n;
+    host_run(threads, [&](unsigned) {
+        for (size_t k; (k = next.fetch_add(1)) < n;) f(k);
+    });
+}
+
+// One host_run call: task indices are claimed by the workers and the caller;
+// every index is claimed and counted, and after a task throws the remaining
+// ones are claimed without running.
+struct HostJob {
+    const std::function<void(unsigned)>* f = nullptr;
+    unsigned T = 0;
+    std::atomic<unsigned> next{0}, done{0};
+    std::atomic<bool> failed{false};
+    std::mutex mu;
+    std::condition_variable cv;
+    std::exception_ptr err;
+
+    void work() {
+        for (unsigned t; (t = next.fetch_add(1)) < T;) {
+            if (!failed.load()) {
+                try {
+                    (*f)(t);
+                } catch (...) {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (!err) err = std::current_exception();
+                    failed = true;
+                }
             }
-        });
-    for (auto& x : th) x.join();
-    for (auto& e : err)
-        if (e) std::rethrow_exception(e);
+            if (done.fetch_add(1) + 1 == T) {
+                std::lock_guard<std::mutex> g(mu);
+                cv.notify_all();
+            }
+        }
+    }
+};
+
+// Persistent workers, never destroyed: at process exit they are idle on
+// the condition variable.
+struct HostPool {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::shared_ptr<HostJob>> q;
+    std::vector<std::thread> workers;
+
+    explicit HostPool(unsigned n) {
+        for (unsigned i = 0; i < n; i++) workers.emplace_back([this] { loop(); });
+        for (auto& w : workers) w.detach();
+    }
+    void loop() {
+        for (;;) {
+            std::shared_ptr<HostJob> j;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return !q.empty(); });
+                j = q.front();
+                if (j->next.load() >= j->T) {  // every task taken: drop the job
+                    q.pop_front();
+                    continue;
+                }
+            }
+            j->work();
+        }
+    }
+};
+
+HostPool& host_pool() {
+    static HostPool* p = new HostPool(host_threads() > 1 ? host_threads() - 1 : 0);
+    return *p;
 }
 
 }  // namespace
+
+void host_run(unsigned T, const std::function<void(unsigned)>& f) {
+    if (T == 0) return;
+    if (T == 1) {
+        f(0);
+        return;
+    }
+    auto j = std::make_shared<HostJob>();
+    j->f = &f;
+    j->T = T;
+    HostPool& P = host_pool();
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        P.q.push_back(j);
+    }
+    P.cv.notify_all();
+    j->work();  // the caller takes every task no worker has started
+    {
+        std::unique_lock<std::mutex> lk(j->mu);
+        j->cv.wait(lk, [&] { return j->done.load() == j->T; });
+    }
+    {  // a worker normally drops the finished job; without workers (a forked child) the caller does
+        std::lock_guard<std::mutex> g(P.mu);
+        for (auto it = P.q.begin(); it != P.q.end(); ++it)
+            if (*it == j) {
+                P.q.erase(it);
+                break;
+            }
+    }
+    if (j->err) std::rethrow_exception(j->err);
+}
 
 // ------------------------------------------------------------------ files
 

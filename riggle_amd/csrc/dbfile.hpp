@@ -5,6 +5,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <set>
 #include <string>
@@ -31,6 +32,13 @@ std::string read_file(const std::string& p);
 void write_file(const std::string& p, const void* data, size_t n);
 // Host threads for parallel host work: hardware threads, at most 16.
 unsigned host_threads();
+// Runs f(t) for every t in [0, T) on the library's persistent host workers
+// (host_threads() - 1 of them, started on first use) and the calling thread.
+// The caller also runs every task no worker has taken yet, so concurrent
+// calls from several threads, and calls from inside a task, always finish.
+// After a task throws, tasks not yet started are skipped and the first
+// exception is rethrown to the caller.
+void host_run(unsigned T, const std::function<void(unsigned)>& f);
 
 // ------------------------------------------------------------------ BED
 struct Iv {

@@ -363,15 +363,7 @@ void query_upload(Query& q, const uint32_t* shard, const uint32_t* start, const 
             P.fs[(size_t)r * S + shard[i]]++;
         }
     };
-    auto run = [&](auto&& f) {
-        if (T == 1) {
-            f(0u);
-            return;
-        }
-        std::vector<std::thread> th;
-        for (unsigned t = 0; t < T; t++) th.emplace_back(f, t);
-        for (auto& x : th) x.join();
-    };
+    auto run = [&](auto&& f) { db::host_run(T, f); };
     run(count);
     for (const Part& P : parts) {
         if (P.bad == ~0ull) continue;

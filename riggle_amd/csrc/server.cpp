@@ -179,13 +179,7 @@ struct Server {
             }
         };
         const unsigned T = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(db::host_threads(), n >> 17));
-        if (T == 1) {
-            stage(0, n);
-        } else {
-            std::vector<std::thread> th;
-            for (unsigned t = 0; t < T; t++) th.emplace_back(stage, n * t / T, n * (t + 1) / T);
-            for (auto& x : th) x.join();
-        }
+        db::host_run(T, [&](unsigned t) { stage(n * t / T, n * (t + 1) / T); });
         query_stage_commit(q);
         if (opt.pvalues) {
             std::vector<uint32_t> qf(n);
@@ -381,15 +375,7 @@ struct Server {
         };
         std::vector<Part> parts(T);
         auto range = [&](unsigned t) { return std::make_pair(n * t / T, n * (t + 1) / T); };
-        auto run = [&](auto&& f) {
-            if (T == 1) {
-                f(0u);
-                return;
-            }
-            std::vector<std::thread> th;
-            for (unsigned t = 0; t < T; t++) th.emplace_back(f, t);
-            for (auto& x : th) x.join();
-        };
+        auto run = [&](auto&& f) { db::host_run(T, f); };
         run([&](unsigned t) {
             Part& P = parts[t];
             P.fs.assign(S, 0);

@@ -1,7 +1,9 @@
 """CPU builds with AddressSanitizer and UndefinedBehaviorSanitizer
 (tests/sanitize/Makefile): the library's host-side format code (BED parsing
 over host threads, meta.json, jump tables, result JSON) driven by
-tests/sanitize/host_driver.cpp, and the oracle under its own CPU tests."""
+tests/sanitize/host_driver.cpp, and the oracle under its own CPU tests; and
+with ThreadSanitizer: the library's host worker pool driven by
+tests/sanitize/pool_stress.cpp."""
 import gzip
 import os
 import subprocess
@@ -65,3 +67,11 @@ def test_oracle_under_sanitizers(built):
                        cwd=ROOT, capture_output=True, text=True, env=env, timeout=900)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     assert "ERROR: AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr
+
+
+def test_host_pool_under_thread_sanitizer(built):
+    env = dict(os.environ)
+    env["TSAN_OPTIONS"] = "halt_on_error=1"
+    r = subprocess.run([os.path.join(built, "pool_stress")], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip() == "bad=0" and "ThreadSanitizer" not in r.stderr
