@@ -248,3 +248,36 @@ def test_fully_covered_chromosome_mean(chk, orc):
     assert 0.0 <= v[so[0] + k] < 1e-4
     worst_m, worst_v = compare(ix, orc, db, 1, 1, [size])
     assert worst_m <= REL and worst_v <= REL
+
+
+def test_forward_column_pass_forms_agree(chk, orc):
+    # the persistent forward column pass (default) and the one-block-per-group
+    # kernel (moments.fft_pipe = 1) give the same tables bit for bit.  Shards
+    # of 5,000, 30,000, 100,000 and 300,000 bins: column FFTs of 8 points
+    # (always the per-group kernel), 32 (radix-4 first pass), 128 (radix-2) and
+    # 512 (radix-8; 384 column groups, so persistent blocks take several);
+    # 5 files, so the last signal pair has no odd row
+    rng = np.random.default_rng(45)
+    F, S = 5, 4
+    sizes = np.array([500_000, 3_000_000, 10_000_000, 30_000_000], np.uint64)
+    files = []
+    for _ in range(F):
+        n = int(rng.integers(2000, 6000))
+        sh = rng.integers(0, S, n).astype(np.uint32)
+        st = (rng.random(n) * (sizes[sh].astype(np.float64) - 30_000)).astype(np.uint32)
+        files.append((sh, st, st + rng.integers(1, 30_000, n).astype(np.uint32)))
+    db = flatten(files)
+    tabs = []
+    for pipe in (0, 1):
+        ix = chk.Index(*db, S, F)
+        ix.set_genome(sizes)
+        ix.tune("moments.fft_pipe", pipe)
+        assert ix.tune_get("moments.fft_pipe") == pipe
+        rows, _ = ix.build_moments()
+        assert rows > 0
+        tabs.append(ix.download_moments())
+        if pipe == 0:
+            wm, wv = compare(ix, orc, db, F, S, sizes)
+            assert wm <= REL and wv <= REL, (wm, wv)
+    for x, y in zip(*tabs):
+        assert np.array_equal(x, y)
