@@ -457,6 +457,8 @@ def batched_leg(ix, cfg, F, nshards, D):
             "batched_over_plain": ms_plain / ms_graph,
             "batched_over_plain_dense": ms_plain_dense / ms_graph,
             "autotuned": knobs, "autotune_s": t_tune, "rows_match_single_launches": rows_ok,
+            "autotune_note": "0 keeps the automatic choice: no candidate below beat it",
+            "autotune_candidates": report,
             "autotune_batch": "a second 256-file batch of the same laws (seed + 40); the timed batch "
                               "was not seen by the tuner"}
 
