@@ -89,6 +89,7 @@ double autotune_body(Query& q, uint64_t* d_counts, uint64_t* d_bp, int reps, std
             }
         }
         set(bv);
+        ix.tune_version++;  // the next plan follows the kept value, not the last candidate
     };
     // method threshold (files above it go dense): only matters when the
     // batch has files on both sides of it; 1 << 62 sends every file sparse

@@ -96,6 +96,11 @@ def test_autotune_changes_speed_not_results(chk, orc):
     dc, dbp = ix.alloc(8 * B * F), ix.alloc(8 * B * F)
     best, knobs, report = ix.autotune(qq, dc, dbp, reps=2)
     assert best > 0 and len(report) >= 5
+    # with dense files in the kept plan the dense launch shape is searched
+    # too (it once went unsearched when the last threshold candidate had sent
+    # every file sparse)
+    if qq.info()["dense_files"]:
+        assert any(r.startswith("sweep.items=") for r in report), report
     c, bp = ix.sweep(qq)
     oc, obp = orc.sweep_ranks(db, q, F, B, S)
     assert (c.astype(np.int64) == oc).all() and (bp.astype(np.int64) == obp).all()
